@@ -1,0 +1,121 @@
+"""The sssp and bfs apps: options, gg_main and output, over the C ABI.
+
+Mirrors the reference app surface:
+  * ``Options`` / ``process_prog_opt``  -- apps/sssp/support.cu:6-31, apps/bfs/support.cu:5-25
+  * ``sssp_gg_main`` / ``bfs_gg_main``  -- apps/sssp/kernel.cu:455-480, apps/bfs/kernel.cu:380-393
+  * ``output``                         -- apps/sssp/support.cu:34-53, apps/bfs/support.cu:27-46
+``sssp`` and ``bfs`` run the whole skeleton sequence of skelapp/skel.cu:45-78
+(copy to GPU, gg_main, copy back) and return the distance array.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, fields
+from typing import Optional, Tuple
+
+import numpy as np
+
+from ._lib import EXPAND_LB, EXPAND_TWB, FAR_REFERENCE, FAR_SPLIT, INF, LsgOpts, LsgStats, check, lib
+from .graph import CSRGraph, DeviceGraph
+
+
+@dataclass
+class Options:
+    delta: int = 10000          # -d, apps/sssp/support.cu:10
+    start_node: int = 0         # -s, apps/sssp/support.cu:12
+    enable_lb: bool = False     # -l
+    expand_mode: int = EXPAND_TWB
+    far_mode: int = FAR_REFERENCE
+    block_size: int = 0
+    grid_blocks: int = 0
+    thread_deg: int = 0
+    heavy_deg: int = 0
+    sched: int = 0
+
+    def to_c(self) -> LsgOpts:
+        o = LsgOpts()
+        lib().lsg_opts_default(ctypes.byref(o))
+        for f in fields(self):
+            setattr(o, f.name, int(getattr(self, f.name)))
+        return o
+
+
+def process_prog_opt(opts: Options, c: str, optarg: Optional[str] = None, app: str = "sssp") -> None:
+    """Applies one command-line option the way support.cu does; bad values raise ValueError
+    where the reference asserts (apps/sssp/support.cu:22,29)."""
+    if c == "d" and app == "sssp":
+        v = int(optarg)
+        if v <= 0:
+            raise ValueError(f"delta must be > 0 (got {optarg})")
+        opts.delta = v
+    elif c == "l":
+        opts.enable_lb = True
+    elif c == "s":
+        v = int(optarg)
+        if v < 0:
+            raise ValueError(f"start node must be >= 0 (got {optarg})")
+        opts.start_node = v
+    else:
+        raise ValueError(f"unknown option -{c} for {app}")
+
+
+def sssp_gg_main(host: Optional[CSRGraph], dev: DeviceGraph, opts: Options) -> dict:
+    st = LsgStats()
+    hg = ctypes.byref(host._g) if host is not None else None
+    check(lib().lsg_sssp_gg_main(hg, ctypes.byref(dev._g), ctypes.byref(opts.to_c()), ctypes.byref(st)),
+          "sssp gg_main")
+    return st.as_dict()
+
+
+def bfs_gg_main(host: Optional[CSRGraph], dev: DeviceGraph, opts: Options) -> dict:
+    st = LsgStats()
+    hg = ctypes.byref(host._g) if host is not None else None
+    check(lib().lsg_bfs_gg_main(hg, ctypes.byref(dev._g), ctypes.byref(opts.to_c()), ctypes.byref(st)),
+          "bfs gg_main")
+    return st.as_dict()
+
+
+def sssp_into(dev: DeviceGraph, dist_ptr: int, opts: Options) -> dict:
+    """gg_main writing into a caller-owned device array of dev.nnodes int32."""
+    st = LsgStats()
+    check(lib().lsg_sssp_dist(ctypes.byref(dev._g), ctypes.byref(opts.to_c()), ctypes.c_void_p(dist_ptr),
+                              ctypes.byref(st)), "sssp")
+    return st.as_dict()
+
+
+def bfs_into(dev: DeviceGraph, level_ptr: int, opts: Options) -> dict:
+    st = LsgStats()
+    check(lib().lsg_bfs_dist(ctypes.byref(dev._g), ctypes.byref(opts.to_c()), ctypes.c_void_p(level_ptr),
+                             ctypes.byref(st)), "bfs")
+    return st.as_dict()
+
+
+def _run(graph: CSRGraph, opts: Options, device: int, fn) -> Tuple[np.ndarray, dict]:
+    dev = graph.copy_to_gpu(device)
+    try:
+        stats = fn(graph, dev, opts)
+        dev.copy_to_cpu(graph)
+    finally:
+        dev.close()
+    return graph.node_data.copy(), stats
+
+
+def sssp(graph: CSRGraph, start_node: int = 0, delta: int = 10000, device: int = 0,
+         **tuning) -> Tuple[np.ndarray, dict]:
+    """Skeleton sequence for apps/sssp: copy_to_gpu, gg_main, copy_to_cpu.  Returns (dist, stats)."""
+    return _run(graph, Options(delta=delta, start_node=start_node, **tuning), device, sssp_gg_main)
+
+
+def bfs(graph: CSRGraph, start_node: int = 0, device: int = 0, **tuning) -> Tuple[np.ndarray, dict]:
+    return _run(graph, Options(start_node=start_node, **tuning), device, bfs_gg_main)
+
+
+def output(dist: np.ndarray, path: Optional[str]) -> None:
+    """Writes "%d %d" / "%d INF" lines (apps/sssp/support.cu:45-51)."""
+    d = np.ascontiguousarray(dist, dtype=np.int32)
+    check(lib().lsg_output_dist(d.ctypes.data_as(ctypes.c_void_p), len(d),
+                                None if path is None else str(path).encode()), "output")
+
+
+__all__ = ["Options", "process_prog_opt", "sssp_gg_main", "bfs_gg_main", "sssp_into", "bfs_into", "sssp",
+           "bfs", "output", "INF", "EXPAND_TWB", "EXPAND_LB", "FAR_REFERENCE", "FAR_SPLIT"]

@@ -1,0 +1,159 @@
+// The sssp / bfs command line, built twice (-DLSG_APP_SSSP / -DLSG_APP_BFS).
+//
+// Mirrors the reference's skeleton driver and app options:
+//   skelapp/skel.cu:80-151   usage, getopt "g:qo:" + app options, main
+//   skelapp/skel.cu:45-78    load graph, copy to GPU, timed gg_main, copy back, output
+//   apps/sssp/support.cu:6-31   "ls:d:" options, DELTA 10000, start node 0
+//   apps/bfs/support.cu:5-25    "ls:" options
+// Invocation: sssp [-q] [-g gpunum] [-o output-file] [-l] [-d delta] [-s startNode] graph-file
+// Reference behaviours kept: the same messages on stdout/stderr, "-o -" for
+// stdout, no output file when -o is absent, exit 1 with the usage text on bad
+// options.  Differences: errors from the library are reported with a message
+// and exit status 1 instead of assert()/crash (negative delta, source out of
+// range, unreadable or malformed graph, unwritable output).
+
+#include <getopt.h>
+#include <time.h>
+
+#include <cerrno>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+#include "lsg.h"
+
+#if defined(LSG_APP_SSSP)
+static const char *prog_opts = "ls:d:";
+static const char *prog_usage = "[-l] [-d delta] [-s startNode]";
+static const char *GGC_OPTIONS = "lsg-mi355x sssp: persistent near/far, wave64, gfx950";
+#elif defined(LSG_APP_BFS)
+static const char *prog_opts = "ls:";
+static const char *prog_usage = "[-l] [-s startNode]";
+static const char *GGC_OPTIONS = "lsg-mi355x bfs: persistent level-synchronous, wave64, gfx950";
+#else
+#error "define LSG_APP_SSSP or LSG_APP_BFS"
+#endif
+static const char *prog_args_usage = "-l: enable thread block load balancer (by default false)";
+
+static int QUIET = 0;
+static char *INPUT = nullptr, *OUTPUT = nullptr;
+static int CUDA_DEVICE = 0;
+static lsg_opts OPTS;
+
+static void usage(int, char *argv[]) {
+  fprintf(stderr, "usage: %s [-q] [-g gpunum] [-o output-file] %s graph-file \n %s\n", argv[0], prog_usage,
+          prog_args_usage);
+}
+
+static void die(const char *what, int rc) {
+  fprintf(stderr, "%s: %s (error %d)\n", what, lsg_last_error(), rc);
+  exit(1);
+}
+
+static void process_prog_opt(char c, char *optarg) {
+  if (c == 'd') {
+    OPTS.delta = atoi(optarg);
+    if (OPTS.delta <= 0) {
+      fprintf(stderr, "delta must be > 0 (got '%s')\n", optarg);
+      exit(EXIT_FAILURE);
+    }
+  }
+  if (c == 'l') OPTS.enable_lb = 1;
+  if (c == 's') {
+    OPTS.start_node = atoi(optarg);
+    if (OPTS.start_node < 0) {
+      fprintf(stderr, "start node must be >= 0 (got '%s')\n", optarg);
+      exit(EXIT_FAILURE);
+    }
+  }
+}
+
+static void parse_args(int argc, char *argv[]) {
+  char opts[64];
+  snprintf(opts, sizeof opts, "g:qo:%s", prog_opts);
+  int c;
+  while ((c = getopt(argc, argv, opts)) != -1) {
+    switch (c) {
+      case 'q': QUIET = 1; break;
+      case 'o': OUTPUT = optarg; break;
+      case 'g': {
+        char *end;
+        errno = 0;
+        CUDA_DEVICE = (int)strtol(optarg, &end, 10);
+        if (errno != 0 || *end != '\0') {
+          fprintf(stderr, "Invalid GPU device '%s'. An integer must be specified.\n", optarg);
+          exit(EXIT_FAILURE);
+        }
+        break;
+      }
+      case '?': usage(argc, argv); exit(EXIT_FAILURE);
+      default: process_prog_opt((char)c, optarg); break;
+    }
+  }
+  if (optind < argc) INPUT = argv[optind];
+  else {
+    usage(argc, argv);
+    exit(EXIT_FAILURE);
+  }
+}
+
+static uint64_t now_ns() {
+  timespec ts;
+  clock_gettime(CLOCK_MONOTONIC_RAW, &ts);  // rt/include/Timer.h:23-26
+  return (uint64_t)ts.tv_sec * 1000000000ull + (uint64_t)ts.tv_nsec;
+}
+
+static int load_graph_and_run_kernel(char *graph_file) {
+  lsg_graph g, gg;
+  fprintf(stderr, "OPTIONS: %s\n", GGC_OPTIONS);
+  int rc = lsg_graph_read(&g, graph_file, 0);
+  if (rc) die("reading graph", rc);
+  rc = lsg_graph_copy_to_gpu(&g, &gg, CUDA_DEVICE);
+  if (rc) die("copy_to_gpu", rc);
+  lsg_stats st;
+#if defined(LSG_APP_SSSP)
+  printf("delta: %d\n", OPTS.delta);  // apps/sssp/kernel.cu:473
+#endif
+  uint64_t t0 = now_ns();
+#if defined(LSG_APP_SSSP)
+  rc = lsg_sssp_gg_main(&g, &gg, &OPTS, &st);
+#else
+  rc = lsg_bfs_gg_main(&g, &gg, &OPTS, &st);
+#endif
+  if (rc == LSG_OK) rc = lsg_device_sync(CUDA_DEVICE);
+  uint64_t t1 = now_ns();
+  if (rc) die("gg_main", rc);
+#if defined(LSG_APP_SSSP)
+  printf("iterations: %d\n", (int)st.iterations);  // apps/sssp/kernel.cu:479
+#endif
+  printf("gg_main %llu %llu\n", (unsigned long long)t0, (unsigned long long)t1);  // Timer.h:77-79
+  fprintf(stderr, "Total time: %llu ms\n", (unsigned long long)((t1 - t0) / 1000000ull));
+  fprintf(stderr, "Total time: %llu ns\n", (unsigned long long)(t1 - t0));
+  rc = lsg_graph_copy_to_cpu(&gg, &g);
+  if (rc) die("copy_to_cpu", rc);
+  if (!QUIET) {
+    fflush(stdout);
+    rc = lsg_output(&g, OUTPUT);
+    if (rc) die("output", rc);
+  }
+  lsg_graph_dealloc(&gg);
+  lsg_graph_dealloc(&g);
+  return 0;
+}
+
+int main(int argc, char *argv[]) {
+  if (argc == 1) {
+    usage(argc, argv);
+    exit(1);
+  }
+  lsg_opts_default(&OPTS);
+  parse_args(argc, argv);
+  int rc = lsg_set_device(CUDA_DEVICE);
+  if (rc) die("selecting GPU", rc);
+  char name[256];
+  rc = lsg_device_name(CUDA_DEVICE, name, sizeof name);
+  if (rc) die("querying GPU", rc);
+  printf("Using GPU: %s\n", name);
+  return load_graph_and_run_kernel(INPUT);
+}

@@ -1,0 +1,676 @@
+// Single-source traversal on gfx950: fixed-delta near/far SSSP and
+// level-synchronous BFS in one persistent, cooperatively launched kernel.
+//
+// Reference path: apps/sssp/kernel.cu (gg_main :455, persistent driver
+// gg_main_pipe_1_gpu_gb :355-391, relax operator sssp_kernel_dev :177-306,
+// far-pile dedupe remove_dups_dev :36-66) and apps/bfs/kernel.cu (gg_main
+// :380, driver :312-334, operator bfs_kernel_dev :121-274).
+//
+// Layout in HBM (all u32): row_start[V+1], edge_dst[E], edge_data[E],
+// dist[V]; three item queues of capacity V (in / near / far roles rotate),
+// two heavy-chunk queues, near_stamp[V] and far_stamp[V] used to push a
+// vertex at most once per round (near) and once per phase (far), and a
+// control block.  The reference's queues hold 2E items each with an equal
+// priority array (apps/sssp/kernel.cu:475, rt/include/worklist.h:82-113);
+// deduplicating at push time bounds every queue by V instead.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "engine.h"
+#include "lsg_internal.h"
+
+namespace lsg {
+
+enum class Algo : int { SSSP = 0, BFS = 1 };
+
+struct KArgs {
+  const uint32_t *row_start;
+  const uint32_t *edge_dst;
+  const uint32_t *edge_data;
+  uint32_t *dist;
+  uint32_t *q[3];
+  uint32_t *hq[2];  // heavy chunk queues: (vertex, chunk index) pairs
+  uint32_t *near_stamp;
+  uint32_t *far_stamp;
+  Ctrl *ctrl;
+  uint32_t nnodes;
+  uint32_t qcap;   // capacity of each item queue
+  uint32_t hcap;   // capacity of each heavy queue in chunks
+  uint32_t delta;
+  uint32_t thread_deg;
+  uint32_t heavy_deg;
+  uint32_t expand_mode;
+  uint32_t far_mode;
+  uint64_t spin_limit;
+};
+
+// ---- per-edge operators ---------------------------------------------------
+
+// Relax u->v with candidate nd = d[u] + w.  Same test and atomicMin as
+// apps/sssp/kernel.cu:129-144, with the push decided by whether the atomicMin
+// really lowered d[v] (the reference pushes on the pre-check alone), and the
+// near/far split at curdelta (:134).  Arithmetic is u32: d[u] < 2^31 and
+// w <= INT_MAX never wrap, and a candidate >= INT_MAX never beats INF.
+struct SsspRelax {
+  __device__ __forceinline__ static void edge(const KArgs &a, const Sched &s, uint32_t v, uint32_t nd,
+                                              bool active) {
+    bool imp = false;
+    if (active) {
+      uint32_t dv = a.dist[v];
+      if (nd < dv) imp = nd < atomic_min(a.dist + v, nd);
+    }
+    bool to_near = imp && (int64_t)nd <= s.curdelta;
+    bool to_far = imp && !to_near;
+    if (to_near) {
+      uint32_t stamp = s.round + 1;
+      to_near = a.near_stamp[v] != stamp && atomic_exch(a.near_stamp + v, stamp) != stamp;
+    } else if (to_far) {
+      uint32_t stamp = s.phase + 1;
+      to_far = a.far_stamp[v] != stamp && atomic_exch(a.far_stamp + v, stamp) != stamp;
+    }
+    wave_push(to_near, v, a.q[s.near_q], &a.ctrl->qcount[s.near_q], a.qcap, &a.ctrl->abort);
+    wave_push(to_far, v, a.q[s.far_q], &a.ctrl->qcount[s.far_q], a.qcap, &a.ctrl->abort);
+  }
+};
+
+// BFS visit.  The reference writes LEVEL when it sees INF and pushes, racing
+// benignly (apps/bfs/kernel.cu:194-200).  Here a level is a unit-weight
+// distance lowered with atomicMin, so a vertex reached late (heavy chunks run
+// one round after their vertex is popped) still ends at its hop count; every
+// improvement is pushed to the next level's pile, once per round.
+struct BfsVisit {
+  __device__ __forceinline__ static void edge(const KArgs &a, const Sched &s, uint32_t v, uint32_t nd,
+                                              bool active) {
+    bool imp = false;
+    if (active) {
+      uint32_t dv = a.dist[v];
+      if (nd < dv) imp = nd < atomic_min(a.dist + v, nd);
+    }
+    if (imp) {
+      uint32_t stamp = s.round + 1;
+      imp = a.near_stamp[v] != stamp && atomic_exch(a.near_stamp + v, stamp) != stamp;
+    }
+    wave_push(imp, v, a.q[s.near_q], &a.ctrl->qcount[s.near_q], a.qcap, &a.ctrl->abort);
+  }
+};
+
+template <Algo A>
+__device__ __forceinline__ void relax_edge(const KArgs &a, const Sched &s, uint32_t e, uint32_t du, bool active) {
+  uint32_t v = 0, nd = 0;
+  if (active) {
+    v = a.edge_dst[e];
+    nd = A == Algo::SSSP ? du + a.edge_data[e] : du + 1u;
+  }
+  if (A == Algo::SSSP) SsspRelax::edge(a, s, v, nd, active);
+  else BfsVisit::edge(a, s, v, nd, active);
+}
+
+// ---- block-level helpers ---------------------------------------------------
+
+// Exclusive prefix sum over the block; returns this thread's offset and sets
+// *total.  Wave scans with shuffles, wave totals through LDS.
+template <int BS>
+__device__ __forceinline__ uint32_t block_exclusive_sum(uint32_t x, uint32_t *total, uint32_t *wsum) {
+  constexpr int NW = BS / kWave;
+  uint32_t lane = lane_id(), wid = threadIdx.x / kWave;
+  uint32_t incl = x;
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    uint32_t y = __shfl_up(incl, off);
+    if (lane >= (uint32_t)off) incl += y;
+  }
+  if (lane == kWave - 1) wsum[wid] = incl;
+  __syncthreads();
+  if (wid == 0) {
+    uint32_t w = lane < (uint32_t)NW ? wsum[lane] : 0u;
+    uint32_t wi = w;
+#pragma unroll
+    for (int off = 1; off < NW; off <<= 1) {
+      uint32_t y = __shfl_up(wi, off);
+      if (lane >= (uint32_t)off) wi += y;
+    }
+    if (lane < (uint32_t)NW) wsum[lane] = wi - w;  // exclusive wave offsets
+    if (lane == NW - 1) wsum[NW] = wi;
+  }
+  __syncthreads();
+  uint32_t r = wsum[wid] + incl - x;
+  *total = wsum[NW];
+  __syncthreads();
+  return r;
+}
+
+template <int BS>
+struct TileShared {
+  uint32_t offs[BS];
+  uint32_t start[BS];
+  uint32_t du[BS];
+  uint32_t wsum[BS / 64 + 1];
+};
+
+// One tile of the in-queue: items [t*BS, t*BS+BS).  Loads the row bounds of
+// each popped vertex (the reference's pop + filter, apps/sssp/kernel.cu:201-210),
+// queues heavy vertices as chunks for the next round, and expands the rest by
+// the selected mode.
+template <Algo A, int BS>
+__device__ void process_tile(const KArgs &a, const Sched &s, uint32_t t, TileShared<BS> &sh) {
+  uint32_t i = t * BS + threadIdx.x;
+  bool valid = i < s.in_count;
+  uint32_t u = valid ? a.q[s.in_q][i] : 0u;
+  uint32_t du = 0, rs = 0, deg = 0;
+  if (valid) {
+    du = a.dist[u];
+    if (du == kInf) valid = false;  // INF nodes are never expanded (apps/sssp/kernel.cu:202)
+  }
+  if (valid) {
+    rs = a.row_start[u];
+    deg = a.row_start[u + 1] - rs;
+  }
+  // Heavy vertices: split into ceil(deg / heavy_deg) chunks, processed by
+  // whole blocks in the next round.
+  bool heavy = deg >= a.heavy_deg;
+  uint32_t nch = heavy ? (deg + a.heavy_deg - 1) / a.heavy_deg : 0u;
+  if (__ballot(heavy)) {
+    uint32_t hq = 1u - s.heavy_in;
+    uint32_t base = wave_reserve(nch, &a.ctrl->hcount[hq]);
+    for (uint32_t k = 0; k < nch; ++k) {
+      uint32_t pos = base + k;
+      if (pos < a.hcap) {
+        a.hq[hq][2 * pos] = u;
+        a.hq[hq][2 * pos + 1] = k;
+      } else {
+        st_relaxed(&a.ctrl->abort, kAbortCapacity);
+      }
+    }
+  }
+  if (heavy) deg = 0;
+
+  if (a.expand_mode == LSG_EXPAND_LB) {
+    // Tile-flattened expansion: prefix sum of degrees, then every lane takes
+    // one edge of the flattened tile (the reference's fg scheduler,
+    // apps/sssp/kernel.cu:266-302, at block granularity).
+    uint32_t total;
+    uint32_t off = block_exclusive_sum<BS>(deg, &total, sh.wsum);
+    sh.offs[threadIdx.x] = off;
+    sh.start[threadIdx.x] = rs;
+    sh.du[threadIdx.x] = du;
+    __syncthreads();
+    for (uint32_t base = 0; base < total; base += BS) {
+      uint32_t e = base + threadIdx.x;
+      bool act = e < total;
+      uint32_t ee = 0, d0 = 0;
+      if (act) {
+        uint32_t lo = 0, hi = BS;
+        while (hi - lo > 1) {
+          uint32_t mid = (lo + hi) >> 1;
+          if (sh.offs[mid] <= e) lo = mid;
+          else hi = mid;
+        }
+        ee = sh.start[lo] + (e - sh.offs[lo]);
+        d0 = sh.du[lo];
+      }
+      relax_edge<A>(a, s, ee, d0, act);
+    }
+    __syncthreads();
+  } else {
+    // Degree-binned: one lane per small vertex, the whole wavefront for
+    // medium vertices (the reference's wp scheduler, :218-264, at 64 lanes).
+    bool small = deg < a.thread_deg;
+    uint32_t my = small ? deg : 0u;
+    uint32_t mx = my;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, off));
+    for (uint32_t j = 0; j < mx; ++j) relax_edge<A>(a, s, rs + j, du, j < my);
+    uint64_t med = __ballot(!small && deg > 0);
+    while (med) {
+      uint32_t l = (uint32_t)__builtin_ctzll(med);
+      med &= med - 1;
+      uint32_t ws = readlane(rs, l), wd = readlane(deg, l), wu = readlane(du, l);
+      for (uint32_t j = 0; j < wd; j += kWave) {
+        uint32_t jj = j + lane_id();
+        relax_edge<A>(a, s, ws + jj, wu, jj < wd);
+      }
+    }
+  }
+}
+
+// One heavy chunk: heavy_deg consecutive edges of one vertex, spread over
+// the block (replaces the -l edge-balanced TB_LB pass, apps/sssp/kernel.cu:78-149).
+template <Algo A, int BS>
+__device__ void process_heavy(const KArgs &a, const Sched &s, uint32_t c) {
+  const uint32_t *hq = a.hq[s.heavy_in];
+  uint32_t u = hq[2 * c], k = hq[2 * c + 1];
+  uint32_t du = a.dist[u];
+  uint32_t rs = a.row_start[u], re = a.row_start[u + 1];
+  uint32_t lo = rs + k * a.heavy_deg;
+  uint32_t hi = min(re, lo + a.heavy_deg);
+  for (uint32_t b = lo; b < hi; b += BS) {
+    uint32_t e = b + threadIdx.x;
+    relax_edge<A>(a, s, e, du, e < hi);
+  }
+}
+
+// Leader step between rounds (single thread).  Implements the pile rotation of
+// the persistent driver (apps/sssp/kernel.cu:363-384): iterate the near pile to
+// empty (retry2), then make the far pile the next phase's input (advance2)
+// with curdelta += DELTA; finish when both are empty.
+template <Algo A>
+__device__ void leader_step(const KArgs &a) {
+  Ctrl *c = a.ctrl;
+  Sched s = c->s;
+  uint32_t nnear = ld_relaxed(&c->qcount[s.near_q]);
+  uint32_t nfar = ld_relaxed(&c->qcount[s.far_q]);
+  uint32_t nheavy = ld_relaxed(&c->hcount[1u - s.heavy_in]);
+  if (nnear > a.qcap || nfar > a.qcap || nheavy > a.hcap) {
+    // A push found its queue full (and raised kAbortCapacity); never hand
+    // out a count beyond the buffer.
+    st_relaxed(&c->abort, kAbortCapacity);
+    s.done = 1;
+    c->s = s;
+    return;
+  }
+  c->stat_items += s.in_count;
+  c->stat_heavy += s.heavy_count;
+  c->stat_rounds += 1;
+  uint32_t consumed = s.in_q;
+  st_relaxed(&c->hcount[s.heavy_in], 0u);  // consumed heavy queue becomes the next producer
+  s.heavy_in = 1u - s.heavy_in;
+  s.heavy_count = nheavy;
+  if (nnear > 0 || nheavy > 0) {
+    s.in_q = s.near_q;
+    s.in_count = nnear;
+    s.near_q = consumed;
+    st_relaxed(&c->qcount[consumed], 0u);
+    s.round += 1;
+  } else if (A == Algo::SSSP && nfar > 0) {
+    s.in_q = s.far_q;
+    s.in_count = nfar;
+    s.far_q = consumed;
+    st_relaxed(&c->qcount[consumed], 0u);
+    s.curdelta += s.delta;
+    s.phase += 1;
+    s.round += 1;
+    c->stat_phases += 1;
+  } else {
+    s.done = 1;
+    c->stat_phases += 1;
+  }
+  s.ntiles = 0;
+  c->s = s;
+  st_relaxed(&c->ticket, 0u);
+}
+
+template <Algo A, int BS>
+__global__ __launch_bounds__(BS) void persistent_kernel(KArgs a) {
+  __shared__ TileShared<BS> sh;
+  __shared__ Sched ss;
+  __shared__ uint32_t tk;
+  __shared__ uint32_t quit;
+  Ctrl *c = a.ctrl;
+  const uint32_t nb = gridDim.x;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      Sched s = c->s;
+      s.ntiles = (s.in_count + BS - 1) / BS;
+      ss = s;
+      quit = s.done || ld_relaxed(&c->abort) != kAbortNone;
+    }
+    __syncthreads();
+    if (quit) break;
+    const Sched s = ss;
+    const uint32_t total = s.ntiles + s.heavy_count;
+    // Static first pass (ticket = block id), then dynamic tickets.
+    uint32_t t = blockIdx.x;
+    while (t < total) {
+      if (t < s.ntiles) process_tile<A, BS>(a, s, t, sh);
+      else process_heavy<A, BS>(a, s, t - s.ntiles);
+      if (total <= nb) break;
+      __syncthreads();
+      if (threadIdx.x == 0) tk = nb + atomic_add(&c->ticket, 1u);
+      __syncthreads();
+      t = tk;
+    }
+    grid_barrier(c, nb, a.spin_limit, [&] { leader_step<A>(a); });
+  }
+}
+
+// Initialise distances, stamps and the control block; seed the in-queue with
+// the source (apps/sssp/kernel.cu:23-35,469,476-477; apps/bfs/kernel.cu:20-32).
+__global__ void init_kernel(KArgs a, uint32_t src, uint32_t delta) {
+  uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
+  for (uint32_t v = tid; v < a.nnodes; v += nt) {
+    a.dist[v] = v == src ? 0u : kInf;
+    a.near_stamp[v] = 0u;
+    a.far_stamp[v] = 0u;
+  }
+  if (tid == 0) {
+    Ctrl *c = a.ctrl;
+    c->bar_count = 0;
+    c->bar_gen = 0;
+    c->qcount[0] = 1;
+    c->qcount[1] = 0;
+    c->qcount[2] = 0;
+    c->hcount[0] = 0;
+    c->hcount[1] = 0;
+    c->ticket = 0;
+    c->abort = kAbortNone;
+    Sched s;
+    memset(&s, 0, sizeof s);
+    s.in_q = 0;
+    s.near_q = 1;
+    s.far_q = 2;
+    s.in_count = 1;
+    s.heavy_in = 0;
+    s.heavy_count = 0;
+    s.round = 1;
+    s.phase = 0;
+    s.curdelta = 0;  // apps/sssp/kernel.cu:472
+    s.delta = delta;
+    c->s = s;
+    c->stat_items = c->stat_rounds = c->stat_heavy = c->stat_phases = 0;
+    a.q[0][0] = src;
+  }
+}
+
+// ---- host side ------------------------------------------------------------
+
+#define HIPCHK(expr)                                                                        \
+  do {                                                                                      \
+    hipError_t _e = (expr);                                                                 \
+    if (_e != hipSuccess) return fail(LSG_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(_e)); \
+  } while (0)
+
+// Per-device workspace, grown on demand and kept for later calls.
+struct Workspace {
+  int device = -1;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int cus = 0;
+  double wall_khz = 0;
+  uint32_t cap_nodes = 0, cap_heavy = 0;
+  uint32_t *q = nullptr, *hq = nullptr, *stamps = nullptr;
+  Ctrl *ctrl = nullptr;
+  std::mutex mu;
+};
+
+static std::mutex g_ws_mu;
+static std::vector<Workspace *> g_ws;
+
+static int get_workspace(int device, Workspace **out) {
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  for (auto *w : g_ws)
+    if (w->device == device) {
+      *out = w;
+      return LSG_OK;
+    }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(LSG_ERR_NODEV, "no HIP device available");
+  if (device < 0 || device >= ndev) return fail(LSG_ERR_ARG, "device %d out of range (%d devices)", device, ndev);
+  HIPCHK(hipSetDevice(device));
+  auto *w = new Workspace;
+  w->device = device;
+  HIPCHK(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking));
+  HIPCHK(hipEventCreate(&w->ev0));
+  HIPCHK(hipEventCreate(&w->ev1));
+  HIPCHK(hipDeviceGetAttribute(&w->cus, hipDeviceAttributeMultiprocessorCount, device));
+  int khz = 0;
+  HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device));
+  w->wall_khz = khz > 0 ? khz : 100000.0;
+  HIPCHK(hipMalloc(&w->ctrl, sizeof(Ctrl)));
+  g_ws.push_back(w);
+  *out = w;
+  return LSG_OK;
+}
+
+static int ensure_capacity(Workspace *w, uint32_t nnodes, uint32_t heavy) {
+  if (nnodes <= w->cap_nodes && heavy <= w->cap_heavy) return LSG_OK;
+  uint32_t n = std::max(nnodes, w->cap_nodes), h = std::max(heavy, w->cap_heavy);
+  if (w->q) HIPCHK(hipFree(w->q));
+  if (w->hq) HIPCHK(hipFree(w->hq));
+  if (w->stamps) HIPCHK(hipFree(w->stamps));
+  w->q = w->hq = w->stamps = nullptr;
+  w->cap_nodes = w->cap_heavy = 0;
+  if (hipMalloc(&w->q, 3ull * n * 4) != hipSuccess || hipMalloc(&w->hq, 2ull * 2 * h * 4) != hipSuccess ||
+      hipMalloc(&w->stamps, 2ull * n * 4) != hipSuccess)
+    return fail(LSG_ERR_NOMEM, "device workspace for %u nodes: out of memory", n);
+  w->cap_nodes = n;
+  w->cap_heavy = h;
+  return LSG_OK;
+}
+
+using KernelFn = void (*)(KArgs);
+
+template <Algo A>
+static KernelFn pick_kernel(int bs) {
+  switch (bs) {
+    case 512: return persistent_kernel<A, 512>;
+    case 1024: return persistent_kernel<A, 1024>;
+    default: return persistent_kernel<A, 256>;
+  }
+}
+
+static uint64_t spin_limit_ticks(const Workspace *w) {
+  double sec = 20.0;
+  if (const char *e = getenv("LSG_BARRIER_TIMEOUT_S")) sec = atof(e);
+  if (sec < 0) sec = 0;
+  return (uint64_t)(sec * w->wall_khz * 1000.0);
+}
+
+// Runs one traversal into dist (device array of g->nnodes).  Synchronous.
+template <Algo A>
+static int run_single(const lsg_graph *g, const lsg_opts *o_in, uint32_t *dist, lsg_stats *st) {
+  auto t0 = std::chrono::steady_clock::now();
+  if (!g || !g->device_graph) return fail(LSG_ERR_ARG, "expected a device graph (copy_to_gpu first)");
+  if (!dist) return fail(LSG_ERR_ARG, "NULL distance array");
+  lsg_opts o;
+  if (o_in) o = *o_in;
+  else lsg_opts_default(&o);
+  if (A == Algo::SSSP && o.delta <= 0) return fail(LSG_ERR_ARG, "delta must be > 0 (got %d)", o.delta);
+  if (o.start_node < 0 || (uint32_t)o.start_node >= g->nnodes)
+    return fail(LSG_ERR_ARG, "start node %d out of range [0, %u)", o.start_node, g->nnodes);
+  int bs = o.block_size ? o.block_size : 256;
+  if (bs != 256 && bs != 512 && bs != 1024) return fail(LSG_ERR_ARG, "block_size must be 256, 512 or 1024");
+  uint32_t thread_deg = o.thread_deg > 0 ? (uint32_t)o.thread_deg : 8u;
+  uint32_t heavy_deg = o.heavy_deg > 0 ? (uint32_t)o.heavy_deg : 4096u;
+  if (heavy_deg < (uint32_t)bs) heavy_deg = (uint32_t)bs;
+  int expand = o.enable_lb ? LSG_EXPAND_LB : o.expand_mode;
+  if (expand != LSG_EXPAND_TWB && expand != LSG_EXPAND_LB) return fail(LSG_ERR_ARG, "bad expand_mode %d", expand);
+
+  Workspace *w;
+  int rc = get_workspace(g->device, &w);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(w->mu);
+  HIPCHK(hipSetDevice(w->device));
+  uint64_t hcap64 = 2ull * g->nedges / heavy_deg + 64;
+  uint32_t hcap = (uint32_t)std::min<uint64_t>(hcap64, UINT32_MAX / 2);
+  rc = ensure_capacity(w, g->nnodes, hcap);
+  if (rc) return rc;
+
+  KArgs a;
+  memset(&a, 0, sizeof a);
+  a.row_start = g->row_start;
+  a.edge_dst = g->edge_dst;
+  a.edge_data = (const uint32_t *)g->edge_data;
+  a.dist = dist;
+  for (int i = 0; i < 3; ++i) a.q[i] = w->q + (size_t)i * w->cap_nodes;
+  a.hq[0] = w->hq;
+  a.hq[1] = w->hq + 2ull * w->cap_heavy;
+  a.near_stamp = w->stamps;
+  a.far_stamp = w->stamps + w->cap_nodes;
+  a.ctrl = w->ctrl;
+  a.nnodes = g->nnodes;
+  a.qcap = w->cap_nodes;
+  a.hcap = w->cap_heavy;
+  a.delta = A == Algo::SSSP ? (uint32_t)o.delta : 1u;
+  a.thread_deg = thread_deg;
+  a.heavy_deg = heavy_deg;
+  a.expand_mode = (uint32_t)expand;
+  a.far_mode = (uint32_t)o.far_mode;
+  a.spin_limit = spin_limit_ticks(w);
+
+  KernelFn fn = A == Algo::SSSP ? pick_kernel<Algo::SSSP>(bs) : pick_kernel<Algo::BFS>(bs);
+  int per_cu = 0;
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)fn, bs, 0));
+  if (per_cu <= 0) return fail(LSG_ERR_HIP, "persistent kernel cannot be resident (block %d)", bs);
+  int max_grid = per_cu * w->cus;
+  int grid = o.grid_blocks > 0 ? std::min(o.grid_blocks, max_grid) : max_grid;
+
+  HIPCHK(hipEventRecord(w->ev0, w->stream));
+  uint32_t init_blocks = std::min<uint32_t>((g->nnodes + 255) / 256, 8u * w->cus);
+  hipLaunchKernelGGL(init_kernel, dim3(init_blocks), dim3(256), 0, w->stream, a, (uint32_t)o.start_node, a.delta);
+  HIPCHK(hipGetLastError());
+  void *args[] = {&a};
+  HIPCHK(hipLaunchCooperativeKernel((const void *)fn, dim3(grid), dim3(bs), args, 0, w->stream));
+  HIPCHK(hipEventRecord(w->ev1, w->stream));
+  Ctrl hc;
+  HIPCHK(hipMemcpyAsync(&hc, w->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, w->stream));
+  HIPCHK(hipStreamSynchronize(w->stream));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, w->ev0, w->ev1));
+  if (hc.abort == kAbortTimeout)
+    return fail(LSG_ERR_TIMEOUT, "grid barrier timed out (grid %d x %d); distances are incomplete", grid, bs);
+  if (hc.abort == kAbortCapacity) return fail(LSG_ERR_CAPACITY, "device queue capacity exceeded");
+  if (st) {
+    memset(st, 0, sizeof *st);
+    st->ms_kernel = ms;
+    st->iterations = A == Algo::SSSP ? hc.stat_phases : hc.stat_rounds;
+    st->near_iters = hc.stat_rounds;
+    st->items = hc.stat_items;
+    st->heavy_chunks = hc.stat_heavy;
+    st->last_delta = (int32_t)std::min<int64_t>(hc.s.curdelta, INT_MAX);
+    st->grid_blocks = grid;
+    st->block_size = bs;
+    st->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  return LSG_OK;
+}
+
+}  // namespace lsg
+
+using namespace lsg;
+
+extern "C" {
+
+int lsg_set_device(int device) {
+  HIPCHK(hipSetDevice(device));
+  return LSG_OK;
+}
+
+int lsg_device_name(int device, char *buf, size_t len) {
+  if (!buf || len == 0) return fail(LSG_ERR_ARG, "device_name: no buffer");
+  hipDeviceProp_t p;
+  HIPCHK(hipGetDeviceProperties(&p, device));
+  snprintf(buf, len, "%s (%s, %d CUs)", p.name[0] ? p.name : "AMD GPU", p.gcnArchName, p.multiProcessorCount);
+  return LSG_OK;
+}
+
+int lsg_device_sync(int device) {
+  HIPCHK(hipSetDevice(device));
+  HIPCHK(hipDeviceSynchronize());
+  return LSG_OK;
+}
+
+int lsg_graph_copy_to_gpu(const lsg_graph *h, lsg_graph *d, int device) {
+  if (!h || !d) return fail(LSG_ERR_ARG, "copy_to_gpu: NULL graph");
+  int rc = validate_host_csr(h);
+  if (rc) return rc;
+  memset(d, 0, sizeof *d);
+  HIPCHK(hipSetDevice(device));
+  size_t ne = h->nedges ? h->nedges : 1;
+  if (hipMalloc(&d->row_start, ((size_t)h->nnodes + 1) * 4) != hipSuccess ||
+      hipMalloc(&d->edge_dst, ne * 4) != hipSuccess || hipMalloc(&d->edge_data, ne * 4) != hipSuccess ||
+      hipMalloc(&d->node_data, (size_t)h->nnodes * 4) != hipSuccess) {
+    (void)hipFree(d->row_start);
+    (void)hipFree(d->edge_dst);
+    (void)hipFree(d->edge_data);
+    (void)hipFree(d->node_data);
+    memset(d, 0, sizeof *d);
+    return fail(LSG_ERR_NOMEM, "copy_to_gpu: device allocation failed");
+  }
+  d->nnodes = h->nnodes;
+  d->nedges = h->nedges;
+  d->device_graph = 1;
+  d->device = device;
+  HIPCHK(hipMemcpy(d->edge_dst, h->edge_dst, (size_t)h->nedges * 4, hipMemcpyHostToDevice));
+  if (h->edge_data) HIPCHK(hipMemcpy(d->edge_data, h->edge_data, (size_t)h->nedges * 4, hipMemcpyHostToDevice));
+  else HIPCHK(hipMemset(d->edge_data, 0, (size_t)h->nedges * 4));
+  if (h->node_data) HIPCHK(hipMemcpy(d->node_data, h->node_data, (size_t)h->nnodes * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d->row_start, h->row_start, ((size_t)h->nnodes + 1) * 4, hipMemcpyHostToDevice));
+  return LSG_OK;
+}
+
+int lsg_graph_copy_to_cpu(const lsg_graph *d, lsg_graph *h) {
+  if (!d || !h || !d->device_graph || h->device_graph) return fail(LSG_ERR_ARG, "copy_to_cpu: need device -> host");
+  if (d->nnodes != h->nnodes) return fail(LSG_ERR_ARG, "copy_to_cpu: node counts differ");
+  if (!d->node_data || !h->node_data) return fail(LSG_ERR_ARG, "copy_to_cpu: missing node_data");
+  HIPCHK(hipSetDevice(d->device));
+  HIPCHK(hipMemcpy(h->node_data, d->node_data, (size_t)d->nnodes * 4, hipMemcpyDeviceToHost));
+  return LSG_OK;
+}
+
+void lsg_graph_dealloc(lsg_graph *g) {
+  if (!g) return;
+  if (g->device_graph == 1) {
+    (void)hipSetDevice(g->device);
+    (void)hipFree(g->row_start);
+    (void)hipFree(g->edge_dst);
+    (void)hipFree(g->edge_data);
+    (void)hipFree(g->node_data);
+    memset(g, 0, sizeof *g);
+  } else if (g->device_graph == 0) {
+    free(g->row_start);
+    free(g->edge_dst);
+    free(g->edge_data);
+    free(g->node_data);
+    memset(g, 0, sizeof *g);
+  } else {
+    memset(g, 0, sizeof *g);  // wrapped caller-owned arrays
+  }
+}
+
+int lsg_graph_wrap_device(lsg_graph *d, uint32_t nnodes, uint32_t nedges, uint32_t *row_start, uint32_t *edge_dst,
+                          int32_t *edge_data, int32_t *node_data, int device) {
+  if (!d || !row_start || (nedges && (!edge_dst || !edge_data)) || nnodes == 0)
+    return fail(LSG_ERR_ARG, "wrap_device: bad arguments");
+  memset(d, 0, sizeof *d);
+  d->nnodes = nnodes;
+  d->nedges = nedges;
+  d->row_start = row_start;
+  d->edge_dst = edge_dst;
+  d->edge_data = edge_data;
+  d->node_data = node_data;
+  d->device_graph = 2;  // device arrays owned by the caller
+  d->device = device;
+  return LSG_OK;
+}
+
+int lsg_sssp_dist(const lsg_graph *gg, const lsg_opts *opts, int32_t *dist_dev, lsg_stats *stats) {
+  clear_error();
+  return run_single<Algo::SSSP>(gg, opts, (uint32_t *)dist_dev, stats);
+}
+
+int lsg_bfs_dist(const lsg_graph *gg, const lsg_opts *opts, int32_t *level_dev, lsg_stats *stats) {
+  clear_error();
+  return run_single<Algo::BFS>(gg, opts, (uint32_t *)level_dev, stats);
+}
+
+int lsg_sssp_gg_main(lsg_graph *hg, lsg_graph *gg, const lsg_opts *opts, lsg_stats *stats) {
+  (void)hg;
+  if (!gg) return fail(LSG_ERR_ARG, "gg_main: NULL device graph");
+  return lsg_sssp_dist(gg, opts, gg->node_data, stats);
+}
+
+int lsg_bfs_gg_main(lsg_graph *hg, lsg_graph *gg, const lsg_opts *opts, lsg_stats *stats) {
+  (void)hg;
+  if (!gg) return fail(LSG_ERR_ARG, "gg_main: NULL device graph");
+  return lsg_bfs_dist(gg, opts, gg->node_data, stats);
+}
+
+}  // extern "C"

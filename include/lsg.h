@@ -1,0 +1,182 @@
+/*
+ * lsg.h -- C ABI of the MI355X (gfx950) worklist shortest-path library.
+ *
+ * This is the drop-in boundary for LonestarGPU's apps/sssp and apps/bfs path.
+ * The reference's boundary is a link-time plugin contract between the CLI
+ * skeleton and each app (skelapp/skel.cu:12-32): the skeleton reads a .gr file
+ * into a host CSRGraph, copies it to the device, times the app's gg_main and
+ * prints the result with the app's output().  Every entry point below replaces
+ * one piece of that contract and cites the reference function it stands in
+ * for.  All entry points are plain C: pointers, sizes and fixed-width ints, no
+ * HIP or torch types.  They return LSG_OK (0) or a negative LSG_ERR_* code
+ * and never call exit(); lsg_last_error() gives a message for the calling
+ * thread.
+ *
+ * Distances and levels are int32 with INF = INT_MAX, exactly as the reference
+ * stores them (apps/sssp/kernel.cu:15-18, apps/bfs/kernel.cu:14-16).
+ */
+#ifndef LSG_H
+#define LSG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LSG_INF 0x7FFFFFFF /* INT_MAX, apps/sssp/kernel.cu:18 */
+#define LSG_ABI_VERSION 1
+
+enum {
+  LSG_OK = 0,
+  LSG_ERR_ARG = -1,      /* bad argument (bad source, delta <= 0, NULL pointer) */
+  LSG_ERR_IO = -2,       /* open/read/write/mmap failure */
+  LSG_ERR_FORMAT = -3,   /* malformed .gr file or graph arrays */
+  LSG_ERR_NOMEM = -4,    /* host or device allocation failed */
+  LSG_ERR_HIP = -5,      /* HIP runtime error */
+  LSG_ERR_CAPACITY = -6, /* a device queue would overflow (never silently dropped) */
+  LSG_ERR_TIMEOUT = -7,  /* a grid barrier exceeded its spin limit; results invalid */
+  LSG_ERR_NODEV = -8     /* no usable gfx950 device */
+};
+
+/* Mirrors struct CSRGraph (rt/include/csr_graph.h:22-24,96-101): index_type is
+ * unsigned, edge_data_type and node_data_type are int.  row_start has
+ * nnodes+1 entries, row_start[nnodes] == nedges.  When device_graph is 1 the
+ * four arrays are device pointers on HIP device `device`. */
+typedef struct lsg_graph {
+  uint32_t nnodes, nedges;
+  uint32_t *row_start;
+  uint32_t *edge_dst;
+  int32_t *edge_data;
+  int32_t *node_data;
+  int32_t device_graph;
+  int32_t device;
+} lsg_graph;
+
+/* Expansion modes: how a frontier vertex's edges are spread over lanes. */
+enum {
+  LSG_EXPAND_TWB = 0, /* by degree: thread (< thread_deg), wavefront (< heavy_deg),
+                         heavy vertices split into block chunks */
+  LSG_EXPAND_LB = 1   /* tile-flattened: block prefix sum of degrees, every lane
+                         takes one edge (the reference's fine-grained np) */
+};
+
+/* Far-pile modes. */
+enum {
+  LSG_FAR_REFERENCE = 0, /* whole far pile becomes the next phase (apps/sssp/kernel.cu:374-383) */
+  LSG_FAR_SPLIT = 1      /* far items above the next threshold wait for a later phase */
+};
+
+/* Options of the sssp/bfs apps: -d, -s and -l (apps/sssp/support.cu:6-31,
+ * apps/bfs/support.cu:5-25) plus the schedule knobs the autotuner and the
+ * learned scheduler set.  Zero means "default" for every tuning field. */
+typedef struct lsg_opts {
+  int32_t delta;       /* -d, sssp only; default 10000 (apps/sssp/support.cu:10) */
+  int32_t start_node;  /* -s; default 0 (apps/sssp/support.cu:12) */
+  int32_t enable_lb;   /* -l: heavy vertices (degree >= total threads) are edge-balanced */
+  int32_t expand_mode; /* LSG_EXPAND_* */
+  int32_t far_mode;    /* LSG_FAR_* */
+  int32_t block_size;  /* threads per block: 256 (default), 512 or 1024 */
+  int32_t grid_blocks; /* persistent grid; 0 = every block the device can keep resident */
+  int32_t thread_deg;  /* TWB: degrees below this are expanded by one lane (default 8) */
+  int32_t heavy_deg;   /* degrees >= this are split into chunks of heavy_deg edges (default 4096) */
+  int32_t sched;       /* 0 = fixed delta, 1 = learned per-phase delta (lsg_sched_set_model) */
+  int32_t reserved[6];
+} lsg_opts;
+
+/* Run statistics.  ms_total mirrors the reference's timed region around
+ * gg_main (skelapp/skel.cu:60-63), which includes worklist allocation
+ * (apps/sssp/kernel.cu:459,475); ms_kernel is device time of the traversal
+ * kernels only (HIP events on the library's stream). */
+typedef struct lsg_stats {
+  double ms_total;
+  double ms_kernel;
+  uint64_t iterations;  /* sssp: delta phases ("iterations: %d", apps/sssp/kernel.cu:479);
+                           bfs: levels */
+  uint64_t near_iters;  /* queue rounds over all phases */
+  uint64_t items;       /* frontier items popped */
+  uint64_t heavy_chunks;
+  int32_t last_delta;   /* curdelta at exit */
+  int32_t grid_blocks;
+  int32_t block_size;
+  int32_t reserved[5];
+} lsg_stats;
+
+/* ---- version / errors ---------------------------------------------------- */
+int lsg_abi_version(void);
+const char *lsg_last_error(void);
+void lsg_opts_default(lsg_opts *o);
+
+/* ---- graph container (rt/src/csr_graph.cu) -------------------------------- */
+/* CSRGraph::read -> readFromGR (rt/src/csr_graph.cu:152-241): reads a .gr v1
+ * file into a host graph (arrays malloc'ed, node_data zeroed).  Unlike the
+ * reference, which only warns about dst >= nnodes (:219) and asserts on the
+ * version (:189), every inconsistency is an LSG_ERR_FORMAT. Weights above
+ * INT_MAX are rejected too. `quiet` suppresses the reference's progress prints. */
+int lsg_graph_read(lsg_graph *g, const char *path, int quiet);
+/* .gr v1 writer, the inverse of lsg_graph_read (byte layout of
+ * rt/src/csr_graph.cu:187-199).  with_weights = 0 writes sizeEdgeTy = 0. */
+int lsg_graph_write(const lsg_graph *g, const char *path, int with_weights);
+/* CSRGraph::allocOnHost (rt/src/csr_graph.cu:28-46): calloc'ed host arrays. */
+int lsg_graph_alloc_host(lsg_graph *g, uint32_t nnodes, uint32_t nedges);
+/* CSRGraph::copy_to_gpu (rt/src/csr_graph.cu:250-261): allocates `dev` on HIP
+ * device `device` and copies the four arrays.  Validates the CSR first. */
+int lsg_graph_copy_to_gpu(const lsg_graph *host, lsg_graph *dev, int device);
+/* CSRGraph::copy_to_cpu (rt/src/csr_graph.cu:263-275).  Only node_data can
+ * change on the device, so only node_data is copied back. */
+int lsg_graph_copy_to_cpu(const lsg_graph *dev, lsg_graph *host);
+/* CSRGraph::dealloc (rt/src/csr_graph.cu:243-248). */
+void lsg_graph_dealloc(lsg_graph *g);
+/* Wraps caller-owned device arrays (for example torch tensors) as a device
+ * graph without copying.  node_data may be NULL when only *_dist entry points
+ * are used. Nothing is freed by lsg_graph_dealloc for such graphs. */
+int lsg_graph_wrap_device(lsg_graph *dev, uint32_t nnodes, uint32_t nedges, uint32_t *row_start,
+                          uint32_t *edge_dst, int32_t *edge_data, int32_t *node_data, int device);
+
+/* ---- synthetic inputs (seeded, deterministic, multi-threaded) ------------- */
+/* R x C 4-neighbour grid, both directions, id = r*C + c; weights uniform in
+ * [wlo, whi] drawn per directed edge from `seed`. */
+int lsg_gen_grid(lsg_graph *g, uint32_t rows, uint32_t cols, uint32_t wlo, uint32_t whi, uint64_t seed);
+/* Directed RMAT: ef * 2^scale pairs with quadrant probabilities a, b, c
+ * (d = 1-a-b-c); self loops and duplicate edges removed; rows sorted. */
+int lsg_gen_rmat(lsg_graph *g, uint32_t scale, uint32_t edge_factor, double a, double b, double c,
+                 uint32_t wlo, uint32_t whi, uint64_t seed);
+/* Road-like grid: R x C grid with a fraction `drop` of undirected edges
+ * removed and `shortcut_frac` * (#kept undirected edges) random long
+ * undirected shortcuts added. */
+int lsg_gen_roadlike(lsg_graph *g, uint32_t rows, uint32_t cols, double drop, double shortcut_frac,
+                     uint32_t wlo, uint32_t whi, uint64_t seed);
+/* Vertex of maximum out-degree, lowest id on ties. */
+uint32_t lsg_max_degree_vertex(const lsg_graph *g);
+
+/* ---- the apps (apps/sssp/kernel.cu:455-480, apps/bfs/kernel.cu:380-393) --- */
+/* gg_main of apps/sssp: initialises gg->node_data to (v == src ? 0 : INF) and
+ * runs fixed-delta near/far relaxation to the fixpoint in one persistent
+ * kernel.  `hg` may be NULL (the reference only reads hg.nnodes). */
+int lsg_sssp_gg_main(lsg_graph *hg, lsg_graph *gg, const lsg_opts *opts, lsg_stats *stats);
+/* gg_main of apps/bfs: level-synchronous BFS; source 0, first hop LEVEL 1. */
+int lsg_bfs_gg_main(lsg_graph *hg, lsg_graph *gg, const lsg_opts *opts, lsg_stats *stats);
+/* Same as the gg_main entry points but writing into a caller-owned device
+ * array dist[gg->nnodes] instead of gg->node_data (for callers that keep
+ * several result arrays resident). */
+int lsg_sssp_dist(const lsg_graph *gg, const lsg_opts *opts, int32_t *dist_dev, lsg_stats *stats);
+int lsg_bfs_dist(const lsg_graph *gg, const lsg_opts *opts, int32_t *level_dev, lsg_stats *stats);
+
+/* output() of apps/sssp and apps/bfs (apps/sssp/support.cu:34-53): one line
+ * per node, "%d INF\n" for INT_MAX and "%d %d\n" otherwise.  path NULL writes
+ * nothing; "-" writes stdout.  `g` must be a host graph. */
+int lsg_output(const lsg_graph *g, const char *path);
+/* The same text for an explicit distance array. */
+int lsg_output_dist(const int32_t *dist, uint32_t n, const char *path);
+
+/* Device synchronisation and name, for the CLI's "Using GPU:" line
+ * (skelapp/skel.cu:146-148). */
+int lsg_set_device(int device);
+int lsg_device_name(int device, char *buf, size_t len);
+int lsg_device_sync(int device);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LSG_H */

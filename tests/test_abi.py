@@ -1,0 +1,83 @@
+"""The C ABI: liblsg.so loads, exports every symbol include/lsg.h declares, and
+reports errors without crashing.  No compute call needs a GPU here."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from conftest import ROOT
+from galoisgpu_amd import _lib
+from galoisgpu_amd.apps import Options, process_prog_opt
+
+HEADER = os.path.join(ROOT, "include", "lsg.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(lsg_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.lib()
+    names = declared_functions()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert missing == []
+    # the Python binding declares a signature for every exported entry point
+    assert set(names) <= set(_lib.SIGNATURES), set(names) - set(_lib.SIGNATURES)
+
+
+def test_exported_symbols_via_nm():
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r"\bT (lsg_[a-z0-9_]+)", out))
+    assert set(declared_functions()) <= exported
+
+
+def test_struct_layouts_match_header():
+    assert ctypes.sizeof(_lib.LsgGraph) == 4 + 4 + 4 * 8 + 4 + 4
+    assert ctypes.sizeof(_lib.LsgOpts) == 16 * 4
+    assert ctypes.sizeof(_lib.LsgStats) == 8 * 6 + 4 * 8
+    assert _lib.lib().lsg_abi_version() == 1
+
+
+def test_defaults_mirror_support_cu():
+    o = _lib.LsgOpts()
+    _lib.lib().lsg_opts_default(ctypes.byref(o))
+    assert o.delta == 10000 and o.start_node == 0 and o.enable_lb == 0  # apps/sssp/support.cu:10-13
+
+
+def test_prog_opts():
+    o = Options()
+    process_prog_opt(o, "d", "32")
+    process_prog_opt(o, "s", "5")
+    process_prog_opt(o, "l")
+    assert (o.delta, o.start_node, o.enable_lb) == (32, 5, True)
+    with pytest.raises(ValueError):
+        process_prog_opt(o, "d", "0")
+    with pytest.raises(ValueError):
+        process_prog_opt(o, "s", "-1")
+
+
+def test_host_graph_rejected_by_gg_main():
+    from galoisgpu_amd import CSRGraph
+    from galoisgpu_amd.apps import sssp_gg_main
+    g = CSRGraph.grid(3, 3)
+    with pytest.raises(_lib.LsgError) as e:
+        sssp_gg_main(g, g, Options())  # a host graph where a device graph is required
+    assert e.value.name == "LSG_ERR_ARG"
+
+
+def test_cli_usage_errors():
+    exe = os.path.join(ROOT, "galoisgpu_amd", "bin", "sssp")
+    r = subprocess.run([exe], capture_output=True, text=True)
+    assert r.returncode == 1 and "usage:" in r.stderr and "[-d delta]" in r.stderr
+    r = subprocess.run([exe, "-x", "g.gr"], capture_output=True, text=True)
+    assert r.returncode == 1 and "usage:" in r.stderr
+    r = subprocess.run([exe, "-d", "0", "g.gr"], capture_output=True, text=True)
+    assert r.returncode == 1 and "delta" in r.stderr
+    exe_bfs = os.path.join(ROOT, "galoisgpu_amd", "bin", "bfs")
+    r = subprocess.run([exe_bfs, "-d", "5", "g.gr"], capture_output=True, text=True)
+    assert r.returncode == 1 and "usage:" in r.stderr  # bfs has no -d (apps/bfs/support.cu:5)

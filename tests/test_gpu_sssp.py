@@ -1,0 +1,215 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle and golden vectors.
+
+Integer work, so every comparison is bit-exact.  Sizes up to RMAT-20 compare
+full distance arrays with the oracle's Dijkstra / BFS; larger graphs are
+checked through the O(E) shortest-path certificate.
+"""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import ROOT
+from galoisgpu_amd import CSRGraph, LsgError, Options
+from galoisgpu_amd import apps
+
+pytestmark = pytest.mark.gpu
+
+INF = 0x7FFFFFFF
+
+
+def say(*a):
+    print("[gpu-test]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def run_sssp(g, dev, src, delta, **kw):
+    apps.sssp_gg_main(g, dev, Options(delta=delta, start_node=src, **kw))
+    dev.copy_to_cpu(g)
+    return g.node_data.copy()
+
+
+def run_bfs(g, dev, src, **kw):
+    apps.bfs_gg_main(g, dev, Options(start_node=src, **kw))
+    dev.copy_to_cpu(g)
+    return g.node_data.copy()
+
+
+MODES = [
+    dict(),
+    dict(expand_mode=1),
+    dict(block_size=512),
+    dict(block_size=1024, expand_mode=1),
+    dict(heavy_deg=256, thread_deg=2),
+    dict(heavy_deg=256, expand_mode=1, grid_blocks=7),
+]
+
+
+def test_small_golden(small_graphs):
+    for name, gg in small_graphs.items():
+        g = CSRGraph.from_arrays(gg["row"], gg["col"], gg["w"])
+        dev = g.copy_to_gpu()
+        for mode in MODES:
+            for delta in (1, 5, 32, 10000):
+                say("small", name, mode, delta)
+                d = run_sssp(g, dev, gg["src"], delta, **mode)
+                assert np.array_equal(d, gg["dist"]), (name, mode, delta)
+            d = run_bfs(g, dev, gg["src"], **mode)
+            assert np.array_equal(d, gg["bfs"]), (name, mode)
+        dev.close()
+
+
+def test_grid256_config1(grid256_golden):
+    """Config 1: 256x256 grid, w in [1,100], source 0, delta 32 -- bit-exact against
+    scipy's golden vector and the near/far restatement."""
+    g = CSRGraph.grid(256, 256, 1, 100, 1)
+    dev = g.copy_to_gpu()
+    nf, _ = oracle.nearfar(g.row_start, g.edge_dst, g.edge_data, 0, 32)
+    assert np.array_equal(nf, grid256_golden["dist"])
+    for mode in MODES:
+        say("grid256", mode)
+        d = run_sssp(g, dev, 0, 32, **mode)
+        assert np.array_equal(d, grid256_golden["dist"]), mode
+    b = run_bfs(g, dev, 0)
+    assert np.array_equal(b, grid256_golden["bfs"])
+
+
+@pytest.fixture(scope="module")
+def rmat20():
+    g = CSRGraph.rmat(20, 16, seed=1)
+    return g
+
+
+def test_rmat20_config2_sssp(rmat20):
+    g = rmat20
+    src = g.max_degree_vertex()
+    ref = oracle.dijkstra(g.row_start, g.edge_dst, g.edge_data, src)
+    dev = g.copy_to_gpu()
+    for mode in (dict(), dict(expand_mode=1), dict(block_size=512, heavy_deg=1024)):
+        say("rmat20 sssp", mode)
+        d = run_sssp(g, dev, src, 32, **mode)
+        assert np.array_equal(d, ref), mode
+    # another source, reference default delta
+    src2 = int(np.flatnonzero(g.out_degree() > 0)[12345])
+    ref2 = oracle.dijkstra(g.row_start, g.edge_dst, g.edge_data, src2)
+    say("rmat20 sssp src2", src2)
+    assert np.array_equal(run_sssp(g, dev, src2, 10000), ref2)
+    dev.close()
+
+
+def test_rmat20_config2_bfs(rmat20):
+    g = rmat20
+    src = g.max_degree_vertex()
+    ref, _ = oracle.bfs(g.row_start, g.edge_dst, src)
+    dev = g.copy_to_gpu()
+    for mode in (dict(), dict(expand_mode=1)):
+        say("rmat20 bfs", mode)
+        assert np.array_equal(run_bfs(g, dev, src, **mode), ref), mode
+    dev.close()
+
+
+def test_roadlike_config3():
+    g = CSRGraph.roadlike(2048, 2048, 0.10, 0.001, 1, 10000, seed=1)
+    ref = oracle.dijkstra(g.row_start, g.edge_dst, g.edge_data, 0)
+    dev = g.copy_to_gpu()
+    for delta, mode in ((32, dict()), (10000, dict(expand_mode=1))):
+        say("roadlike", delta, mode)
+        d = run_sssp(g, dev, 0, delta, **mode)
+        assert np.array_equal(d, ref), (delta, mode)
+    dev.close()
+
+
+def test_rmat22_sources_certificate():
+    g = CSRGraph.rmat(22, 16, seed=1)
+    dev = g.copy_to_gpu()
+    rng = np.random.default_rng(4)
+    for src in [g.max_degree_vertex()] + rng.integers(0, g.nnodes, 3).tolist():
+        say("rmat22 src", src)
+        d = run_sssp(g, dev, int(src), 32)
+        assert oracle.certificate(g.row_start, g.edge_dst, g.edge_data, int(src), d) == 0
+    dev.close()
+
+
+def test_edge_cases(tmp_path):
+    # single node, no edges
+    g = CSRGraph.from_arrays([0, 0], [], [])
+    dev = g.copy_to_gpu()
+    assert run_sssp(g, dev, 0, 32).tolist() == [0]
+    assert run_bfs(g, dev, 0).tolist() == [0]
+    dev.close()
+    # source with no out-edges; zero weights (sizeEdgeTy = 0 file)
+    h = CSRGraph.rmat(10, 8, seed=2)
+    sink = int(np.flatnonzero(h.out_degree() == 0)[0])
+    dev = h.copy_to_gpu()
+    d = run_sssp(h, dev, sink, 32)
+    assert d[sink] == 0 and (np.delete(d, sink) == INF).all()
+    with pytest.raises(LsgError) as e:
+        run_sssp(h, dev, h.nnodes, 32)
+    assert e.value.name == "LSG_ERR_ARG"
+    with pytest.raises(LsgError):
+        run_sssp(h, dev, 0, 0)
+    dev.close()
+    p = str(tmp_path / "z.gr")
+    h.write(p, with_weights=False)
+    z = CSRGraph.read(p)
+    dz = z.copy_to_gpu()
+    src = z.max_degree_vertex()
+    d = run_sssp(z, dz, src, 1)
+    b, _ = oracle.bfs(z.row_start, z.edge_dst, src)
+    assert np.array_equal(d == 0, b != INF) and (d[b == INF] == INF).all()
+    dz.close()
+
+
+def test_workspace_reuse_shrinking_inputs():
+    """Calls in a row on graphs of different sizes share the cached device workspace."""
+    for scale in (16, 12, 14, 10):
+        g = CSRGraph.rmat(scale, 8, seed=scale)
+        src = g.max_degree_vertex()
+        say("reuse", scale)
+        dev = g.copy_to_gpu()
+        d = run_sssp(g, dev, src, 32)
+        assert np.array_equal(d, oracle.dijkstra(g.row_start, g.edge_dst, g.edge_data, src))
+        dev.close()
+
+
+def test_barrier_timeout_is_reported(monkeypatch):
+    """A zero spin limit, set from the host, makes the grid barrier give up at once:
+    every wave still exits and the call reports LSG_ERR_TIMEOUT instead of hanging."""
+    g = CSRGraph.rmat(12, 8, seed=5)
+    dev = g.copy_to_gpu()
+    monkeypatch.setenv("LSG_BARRIER_TIMEOUT_S", "0")
+    say("forced barrier timeout")
+    with pytest.raises(LsgError) as e:
+        run_sssp(g, dev, g.max_degree_vertex(), 32)
+    assert e.value.name == "LSG_ERR_TIMEOUT"
+    monkeypatch.delenv("LSG_BARRIER_TIMEOUT_S")
+    d = run_sssp(g, dev, g.max_degree_vertex(), 32)
+    assert oracle.certificate(g.row_start, g.edge_dst, g.edge_data, g.max_degree_vertex(), d) == 0
+    dev.close()
+
+
+def test_cli_output_matches_oracle_text(tmp_path):
+    """./sssp -d 32 -o out graph.gr: the DiffChecker contract (apps/sssp/bmktest2.py:19)."""
+    g = CSRGraph.grid(64, 64, 1, 100, 3)
+    gr = str(tmp_path / "g.gr")
+    g.write(gr)
+    out = str(tmp_path / "out.txt")
+    exe = os.path.join(ROOT, "galoisgpu_amd", "bin", "sssp")
+    say("cli sssp")
+    r = subprocess.run([exe, "-d", "32", "-o", out, gr], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "Using GPU:" in r.stdout and "delta: 32" in r.stdout and "iterations:" in r.stdout
+    assert "Total time:" in r.stderr
+    ref = oracle.dijkstra(g.row_start, g.edge_dst, g.edge_data, 0)
+    want = "".join(("%d INF\n" % i) if x == INF else ("%d %d\n" % (i, x)) for i, x in enumerate(ref))
+    assert open(out).read() == want
+    out2 = str(tmp_path / "bfs.txt")
+    exe = os.path.join(ROOT, "galoisgpu_amd", "bin", "bfs")
+    r = subprocess.run([exe, "-s", "5", "-o", out2, gr], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lv, _ = oracle.bfs(g.row_start, g.edge_dst, 5)
+    want = "".join(("%d INF\n" % i) if x == INF else ("%d %d\n" % (i, x)) for i, x in enumerate(lv))
+    assert open(out2).read() == want
