@@ -67,7 +67,24 @@ class LsgOpts(ctypes.Structure):
         ("thread_deg", c_int32),
         ("heavy_deg", c_int32),
         ("sched", c_int32),
-        ("reserved", c_int32 * 6),
+        ("trace", c_int32),
+        ("reserved", c_int32 * 5),
+    ]
+
+
+class LsgRoundTrace(ctypes.Structure):
+    _fields_ = [
+        ("round", c_uint32),
+        ("phase", c_uint32),
+        ("in_items", c_uint32),
+        ("heavy_chunks", c_uint32),
+        ("near_out", c_uint32),
+        ("far_out", c_uint32),
+        ("heavy_out", c_uint32),
+        ("curdelta_lo", c_uint32),
+        ("curdelta_hi", c_uint32),
+        ("delta", c_uint32),
+        ("t_end", c_uint64),
     ]
 
 
@@ -135,6 +152,7 @@ SIGNATURES = {
     "lsg_bfs_gg_main": (c_int, [POINTER(LsgGraph), POINTER(LsgGraph), POINTER(LsgOpts), POINTER(LsgStats)]),
     "lsg_sssp_dist": (c_int, [POINTER(LsgGraph), POINTER(LsgOpts), c_void_p, POINTER(LsgStats)]),
     "lsg_bfs_dist": (c_int, [POINTER(LsgGraph), POINTER(LsgOpts), c_void_p, POINTER(LsgStats)]),
+    "lsg_last_trace": (c_int, [c_int, POINTER(LsgRoundTrace), c_uint32, POINTER(c_uint32), POINTER(c_double)]),
     "lsg_output": (c_int, [POINTER(LsgGraph), c_char_p]),
     "lsg_output_dist": (c_int, [c_void_p, c_uint32, c_char_p]),
     "lsg_set_device": (c_int, [c_int]),

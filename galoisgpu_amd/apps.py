@@ -31,6 +31,7 @@ class Options:
     thread_deg: int = 0
     heavy_deg: int = 0
     sched: int = 0
+    trace: bool = False
 
     def to_c(self) -> LsgOpts:
         o = LsgOpts()
@@ -108,6 +109,23 @@ def sssp(graph: CSRGraph, start_node: int = 0, delta: int = 10000, device: int =
 
 def bfs(graph: CSRGraph, start_node: int = 0, device: int = 0, **tuning) -> Tuple[np.ndarray, dict]:
     return _run(graph, Options(start_node=start_node, **tuning), device, bfs_gg_main)
+
+
+TRACE_DTYPE = np.dtype([("round", "<u4"), ("phase", "<u4"), ("in_items", "<u4"), ("heavy_chunks", "<u4"),
+                        ("near_out", "<u4"), ("far_out", "<u4"), ("heavy_out", "<u4"), ("curdelta_lo", "<u4"),
+                        ("curdelta_hi", "<u4"), ("delta", "<u4"), ("t_end", "<u8")])
+
+
+def last_trace(device: int = 0) -> Tuple[np.ndarray, float]:
+    """Round records of the last call made with Options(trace=True): (records, tick_hz)."""
+    from ._lib import LsgRoundTrace
+    n = ctypes.c_uint32()
+    hz = ctypes.c_double()
+    check(lib().lsg_last_trace(device, None, 0, ctypes.byref(n), ctypes.byref(hz)), "last_trace")
+    buf = (LsgRoundTrace * max(1, n.value))()
+    check(lib().lsg_last_trace(device, buf, n.value, ctypes.byref(n), ctypes.byref(hz)), "last_trace")
+    arr = np.frombuffer(bytes(buf), dtype=TRACE_DTYPE)[: n.value].copy()
+    return arr, hz.value
 
 
 def output(dist: np.ndarray, path: Optional[str]) -> None:

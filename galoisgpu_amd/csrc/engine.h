@@ -1,6 +1,18 @@
 // Device-side building blocks shared by the single-source and batched
-// traversal kernels: control block layout, wave64 helpers, wave-aggregated
-// queue pushes and the leader grid barrier.
+// traversal kernels: control block layout, coherent loads/stores, wave64
+// helpers, wave-aggregated queue pushes and the leader grid barrier.
+//
+// Coherence on gfx950.  Each XCD has its own L2, and an agent-scope
+// release/acquire fence writes back and invalidates that L2 -- done by every
+// block at every grid barrier it costs ~0.25 us per block per barrier.  So the
+// kernels never rely on fences for visibility: every word that one block
+// writes and another block reads inside the same launch (queue items, the
+// distances read at pop time, schedule words, counters) is accessed with
+// relaxed agent-scope atomics, which the hardware writes and reads through to
+// the coherence point.  Fences are used only for ordering: before a block
+// signals the barrier it waits for its own memory operations to complete
+// (s_waitcnt vmcnt(0)), and after it leaves it keeps later loads below the
+// spin.  The CSR arrays are read-only during a launch and use plain loads.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -11,51 +23,21 @@ namespace lsg {
 constexpr uint32_t kInf = 0x7FFFFFFFu;  // INT_MAX, apps/sssp/kernel.cu:18
 constexpr int kWave = 64;
 
-enum AbortCode : uint32_t { kAbortNone = 0, kAbortTimeout = 1, kAbortCapacity = 2 };
+enum AbortCode : uint32_t { kAbortNone = 0, kAbortTimeout = 1, kAbortCapacity = 2, kAbortResidency = 3 };
 
-// Schedule for the next round, written by the barrier leader and read by
-// every block after the barrier.  Replaces PipeContextT's by-value pile
-// indices (rt/include/pipe.h:29-87) with one authoritative copy.
-struct Sched {
-  uint32_t in_q, near_q, far_q;  // roles of the three item queues
-  uint32_t in_count;             // items in the in-queue this round
-  uint32_t heavy_in;             // heavy chunk queue consumed this round (0/1)
-  uint32_t heavy_count;          // chunks to consume this round
-  uint32_t round;                // 1-based round number (BFS: the LEVEL written)
-  uint32_t phase;                // 0-based delta phase
-  int64_t curdelta;              // near/far threshold (apps/sssp/kernel.cu:134)
-  uint32_t done;
-  uint32_t ntiles;               // in_count rounded up to blocks
-  uint32_t delta;                // width used for the next phase increment
-  uint32_t pad[3];
-};
+// ---- coherent memory access ----------------------------------------------
 
-// Device control block.  Each hot word sits on its own 128-byte line.
-struct Ctrl {
-  uint32_t bar_count; uint32_t _p0[31];
-  uint32_t bar_gen;   uint32_t _p1[31];
-  uint32_t qcount[3]; uint32_t _p2[29];
-  uint32_t hcount[2]; uint32_t _p3[30];
-  uint32_t ticket;    uint32_t _p4[31];
-  uint32_t abort;     uint32_t _p5[31];
-  Sched s;            uint32_t _p6[16];
-  uint64_t stat_items, stat_rounds, stat_heavy, stat_phases;
-  uint32_t _p7[24];
-};
-
-__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
-
-__device__ __forceinline__ uint32_t ld_acquire(const uint32_t *p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ uint32_t ld_relaxed(const uint32_t *p) {
+__device__ __forceinline__ uint32_t ld_coh(const uint32_t *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ void st_relaxed(uint32_t *p, uint32_t v) {
+__device__ __forceinline__ uint64_t ld_coh64(const uint64_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_coh(uint32_t *p, uint32_t v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ void st_release(uint32_t *p, uint32_t v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ void st_coh64(uint64_t *p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ uint32_t atomic_add(uint32_t *p, uint32_t v) {
   return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -66,17 +48,80 @@ __device__ __forceinline__ uint32_t atomic_min(uint32_t *p, uint32_t v) {
 __device__ __forceinline__ uint32_t atomic_exch(uint32_t *p, uint32_t v) {
   return __hip_atomic_exchange(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ uint32_t atomic_cas(uint32_t *p, uint32_t expect, uint32_t v) {
-  __hip_atomic_compare_exchange_strong(p, &expect, v, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-  return expect;
-}
 __device__ __forceinline__ uint64_t atomic_or64(uint64_t *p, uint64_t v) {
   return __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ uint64_t atomic_exch64(uint64_t *p, uint64_t v) {
   return __hip_atomic_exchange(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+
+// Orders this wave's earlier memory operations before its later ones: the
+// compiler may not move accesses across it and the wave waits until every
+// vector memory operation it issued has completed.
+__device__ __forceinline__ void order_release() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_s_waitcnt(0 | (0x7 << 4) | (0xf << 8));  // vmcnt(0)
+}
+__device__ __forceinline__ void order_acquire() {
+  __builtin_amdgcn_s_waitcnt(0 | (0x7 << 4) | (0xf << 8));
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// ---- schedule and control block ------------------------------------------
+
+// Schedule for the next round, written by the barrier leader and read by
+// every block after the barrier.  Replaces PipeContextT's by-value pile
+// indices (rt/include/pipe.h:29-87) with one authoritative copy.  All words
+// are u32 so it can be copied with coherent loads.
+struct Sched {
+  uint32_t in_q, near_q, far_q;  // roles of the three item queues
+  uint32_t in_count;             // items in the in-queue this round
+  uint32_t heavy_in;             // heavy chunk queue consumed this round (0/1)
+  uint32_t heavy_count;          // chunks to consume this round
+  uint32_t round;                // 1-based round number
+  uint32_t phase;                // 0-based delta phase
+  uint32_t curdelta_lo, curdelta_hi;  // near/far threshold (apps/sssp/kernel.cu:134)
+  uint32_t done;
+  uint32_t delta;                // width of the next phase step
+  uint32_t aux[4];               // kernel-specific words
+  __device__ __host__ int64_t curdelta() const { return (int64_t)(((uint64_t)curdelta_hi << 32) | curdelta_lo); }
+  __device__ __host__ void set_curdelta(int64_t v) {
+    curdelta_lo = (uint32_t)(uint64_t)v;
+    curdelta_hi = (uint32_t)((uint64_t)v >> 32);
+  }
+};
+constexpr int kSchedWords = sizeof(Sched) / 4;
+static_assert(sizeof(Sched) == 64, "Sched is 16 words");
+
+__device__ __forceinline__ void sched_load(Sched &s, const Sched *src) {
+  const uint32_t *p = reinterpret_cast<const uint32_t *>(src);
+  uint32_t *d = reinterpret_cast<uint32_t *>(&s);
+#pragma unroll
+  for (int i = 0; i < kSchedWords; ++i) d[i] = ld_coh(p + i);
+}
+__device__ __forceinline__ void sched_store(Sched *dst, const Sched &s) {
+  uint32_t *p = reinterpret_cast<uint32_t *>(dst);
+  const uint32_t *d = reinterpret_cast<const uint32_t *>(&s);
+#pragma unroll
+  for (int i = 0; i < kSchedWords; ++i) st_coh(p + i, d[i]);
+}
+
+// Device control block.  Each hot word sits on its own 128-byte line.
+struct Ctrl {
+  uint32_t bar_count; uint32_t _p0[31];
+  uint32_t bar_gen;   uint32_t _p1[31];
+  uint32_t qcount[3]; uint32_t _p2[29];
+  uint32_t hcount[2]; uint32_t _p3[30];
+  uint32_t ticket;    uint32_t _p4[31];
+  uint32_t abort;     uint32_t checkin; uint32_t resident_seen; uint32_t _p5[29];
+  Sched s;            uint32_t _p6[16];
+  uint64_t stat_items, stat_rounds, stat_heavy, stat_phases;
+  uint32_t _p7[24];
+};
+
+// ---- wave64 helpers --------------------------------------------------------
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
 // Number of set bits of `mask` below this lane (v_mbcnt).
 __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
@@ -102,15 +147,14 @@ __device__ __forceinline__ void wave_push(bool pred, uint32_t item, uint32_t *q,
   base = readlane(base, leader);
   if (pred) {
     uint32_t pos = base + lanes_below(m);
-    if (pos < cap) q[pos] = item;
-    else st_relaxed(abort_flag, kAbortCapacity);
+    if (pos < cap) st_coh(q + pos, item);
+    else st_coh(abort_flag, kAbortCapacity);
   }
 }
 
 // Reserves `n` consecutive slots per lane (n may differ per lane); returns the
-// first slot for this lane.  One atomic per wave.
+// first slot for this lane.  One atomic per wave; call with the whole wave.
 __device__ __forceinline__ uint32_t wave_reserve(uint32_t n, uint32_t *tail) {
-  // inclusive scan of n over the active lanes via shuffles
   uint32_t lane = lane_id();
   uint32_t x = n;
 #pragma unroll
@@ -125,38 +169,80 @@ __device__ __forceinline__ uint32_t wave_reserve(uint32_t n, uint32_t *tail) {
   return base + x - n;
 }
 
-// Leader grid barrier.  Every block arrives on one counter; the last to
-// arrive runs `leader()` (single thread, sees all writes made before the
-// barrier), resets the counter and bumps the generation; the others spin on
-// the generation with acquire loads.  Replaces GlobalBarrier::Sync
-// (rt/include/gbar.cuh:75-121), whose volatile/LOAD_CG idiom is not enough on
-// gfx950, where each XCD has its own L2: arrival is a release and departure
-// an acquire at agent scope.  The spin is bounded by `spin_limit` ticks of the
-// constant-rate wall clock; when exceeded, or when another block has aborted,
-// the block raises kAbortTimeout and returns so that every wave exits.
+// ---- grid barrier ----------------------------------------------------------
+
+// Leader grid barrier.  Every block arrives on one counter; the last to arrive
+// runs `leader()` (single thread), resets the counter and bumps the
+// generation; the others spin on the generation.  Replaces GlobalBarrier::Sync
+// (rt/include/gbar.cuh:75-121).  The spin is bounded by `spin_limit` ticks of
+// the constant-rate wall clock; when exceeded, or when another block has
+// aborted, the block raises kAbortTimeout and returns so every wave exits.
 template <class Leader>
 __device__ __forceinline__ void grid_barrier(Ctrl *c, uint32_t nblocks, uint64_t spin_limit, Leader leader) {
+  order_release();
   __syncthreads();
   if (threadIdx.x == 0) {
-    uint32_t gen = ld_relaxed(&c->bar_gen);
-    uint32_t arrived = __hip_atomic_fetch_add(&c->bar_count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t gen = ld_coh(&c->bar_gen);
+    uint32_t arrived = atomic_add(&c->bar_count, 1u);
     if (arrived == nblocks - 1) {
+      order_acquire();
       leader();
-      st_relaxed(&c->bar_count, 0u);
-      st_release(&c->bar_gen, gen + 1);
+      st_coh(&c->bar_count, 0u);
+      order_release();
+      st_coh(&c->bar_gen, gen + 1);
     } else {
       uint64_t t0 = wall_clock64();
-      while (ld_acquire(&c->bar_gen) == gen) {
-        if (ld_relaxed(&c->abort) != kAbortNone) break;
+      while (ld_coh(&c->bar_gen) == gen) {
+        if (ld_coh(&c->abort) != kAbortNone) break;
         if (wall_clock64() - t0 > spin_limit) {
-          st_relaxed(&c->abort, kAbortTimeout);
+          st_coh(&c->abort, kAbortTimeout);
           break;
         }
-        __builtin_amdgcn_s_sleep(2);
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+  }
+  order_acquire();
+  __syncthreads();
+}
+
+// Residency handshake, run once at kernel start.  A persistent grid is only
+// correct if every block is resident at once; the runtime's occupancy figure
+// has proven optimistic, so each block checks in and waits (bounded by
+// `limit` ticks) for the others.  If they do not all arrive, the first block
+// to give up records how many had and raises kAbortResidency; blocks that
+// start later see the flag and leave.  Returns false when the launch must
+// be abandoned.
+__device__ __forceinline__ bool residency_handshake(Ctrl *c, uint32_t nblocks, uint64_t limit) {
+  __shared__ uint32_t ok;
+  if (threadIdx.x == 0) {
+    ok = 1;
+    if (ld_coh(&c->abort) != kAbortNone) {
+      ok = 0;
+    } else {
+      uint32_t arrived = atomic_add(&c->checkin, 1u) + 1;
+      uint64_t t0 = wall_clock64();
+      while (arrived < nblocks) {
+        if (ld_coh(&c->abort) != kAbortNone) {
+          ok = 0;
+          break;
+        }
+        if (wall_clock64() - t0 > limit) {
+          // first to give up records the count it saw
+          uint32_t expect = kAbortNone;
+          if (__hip_atomic_compare_exchange_strong(&c->abort, &expect, (uint32_t)kAbortResidency,
+                                                   __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            st_coh(&c->resident_seen, arrived);
+          ok = 0;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        arrived = ld_coh(&c->checkin);
       }
     }
   }
   __syncthreads();
+  return ok != 0;
 }
 
 }  // namespace lsg

@@ -49,7 +49,10 @@ struct KArgs {
   uint32_t heavy_deg;
   uint32_t expand_mode;
   uint32_t far_mode;
-  uint64_t spin_limit;
+  uint64_t spin_limit;       // grid barrier spin bound, wall-clock ticks
+  uint64_t residency_limit;  // start-up handshake bound, wall-clock ticks
+  lsg_round_trace *trace;    // per-round records, or null
+  uint32_t trace_cap;
 };
 
 // ---- per-edge operators ---------------------------------------------------
@@ -62,12 +65,15 @@ struct KArgs {
 struct SsspRelax {
   __device__ __forceinline__ static void edge(const KArgs &a, const Sched &s, uint32_t v, uint32_t nd,
                                               bool active) {
+    // The pre-checks read through the cache: a stale d[v] is only ever larger
+    // than the current one (costing at most a redundant atomicMin, whose
+    // return value is authoritative), and a stale stamp is only ever older.
     bool imp = false;
     if (active) {
       uint32_t dv = a.dist[v];
       if (nd < dv) imp = nd < atomic_min(a.dist + v, nd);
     }
-    bool to_near = imp && (int64_t)nd <= s.curdelta;
+    bool to_near = imp && (int64_t)nd <= s.curdelta();
     bool to_far = imp && !to_near;
     if (to_near) {
       uint32_t stamp = s.round + 1;
@@ -163,10 +169,11 @@ template <Algo A, int BS>
 __device__ void process_tile(const KArgs &a, const Sched &s, uint32_t t, TileShared<BS> &sh) {
   uint32_t i = t * BS + threadIdx.x;
   bool valid = i < s.in_count;
-  uint32_t u = valid ? a.q[s.in_q][i] : 0u;
+  uint32_t u = valid ? ld_coh(a.q[s.in_q] + i) : 0u;
   uint32_t du = 0, rs = 0, deg = 0;
+  valid = valid && u < a.nnodes;
   if (valid) {
-    du = a.dist[u];
+    du = ld_coh(a.dist + u);
     if (du == kInf) valid = false;  // INF nodes are never expanded (apps/sssp/kernel.cu:202)
   }
   if (valid) {
@@ -180,14 +187,11 @@ __device__ void process_tile(const KArgs &a, const Sched &s, uint32_t t, TileSha
   if (__ballot(heavy)) {
     uint32_t hq = 1u - s.heavy_in;
     uint32_t base = wave_reserve(nch, &a.ctrl->hcount[hq]);
+    uint64_t *dst = reinterpret_cast<uint64_t *>(a.hq[hq]);
     for (uint32_t k = 0; k < nch; ++k) {
       uint32_t pos = base + k;
-      if (pos < a.hcap) {
-        a.hq[hq][2 * pos] = u;
-        a.hq[hq][2 * pos + 1] = k;
-      } else {
-        st_relaxed(&a.ctrl->abort, kAbortCapacity);
-      }
+      if (pos < a.hcap) st_coh64(dst + pos, ((uint64_t)k << 32) | u);
+      else st_coh(&a.ctrl->abort, kAbortCapacity);
     }
   }
   if (heavy) deg = 0;
@@ -245,10 +249,12 @@ __device__ void process_tile(const KArgs &a, const Sched &s, uint32_t t, TileSha
 // the block (replaces the -l edge-balanced TB_LB pass, apps/sssp/kernel.cu:78-149).
 template <Algo A, int BS>
 __device__ void process_heavy(const KArgs &a, const Sched &s, uint32_t c) {
-  const uint32_t *hq = a.hq[s.heavy_in];
-  uint32_t u = hq[2 * c], k = hq[2 * c + 1];
-  uint32_t du = a.dist[u];
+  uint64_t ent = ld_coh64(reinterpret_cast<const uint64_t *>(a.hq[s.heavy_in]) + c);
+  uint32_t u = (uint32_t)ent, k = (uint32_t)(ent >> 32);
+  if (u >= a.nnodes) return;
+  uint32_t du = ld_coh(a.dist + u);
   uint32_t rs = a.row_start[u], re = a.row_start[u + 1];
+  if ((uint64_t)k * a.heavy_deg >= re - rs) return;
   uint32_t lo = rs + k * a.heavy_deg;
   uint32_t hi = min(re, lo + a.heavy_deg);
   for (uint32_t b = lo; b < hi; b += BS) {
@@ -261,50 +267,66 @@ __device__ void process_heavy(const KArgs &a, const Sched &s, uint32_t c) {
 // the persistent driver (apps/sssp/kernel.cu:363-384): iterate the near pile to
 // empty (retry2), then make the far pile the next phase's input (advance2)
 // with curdelta += DELTA; finish when both are empty.
+__device__ __forceinline__ void stat_add(uint64_t *p, uint64_t v) { st_coh64(p, ld_coh64(p) + v); }
+
 template <Algo A>
 __device__ void leader_step(const KArgs &a) {
   Ctrl *c = a.ctrl;
-  Sched s = c->s;
-  uint32_t nnear = ld_relaxed(&c->qcount[s.near_q]);
-  uint32_t nfar = ld_relaxed(&c->qcount[s.far_q]);
-  uint32_t nheavy = ld_relaxed(&c->hcount[1u - s.heavy_in]);
+  Sched s;
+  sched_load(s, &c->s);
+  uint32_t nnear = ld_coh(&c->qcount[s.near_q]);
+  uint32_t nfar = ld_coh(&c->qcount[s.far_q]);
+  uint32_t nheavy = ld_coh(&c->hcount[1u - s.heavy_in]);
   if (nnear > a.qcap || nfar > a.qcap || nheavy > a.hcap) {
     // A push found its queue full (and raised kAbortCapacity); never hand
     // out a count beyond the buffer.
-    st_relaxed(&c->abort, kAbortCapacity);
+    st_coh(&c->abort, kAbortCapacity);
     s.done = 1;
-    c->s = s;
+    sched_store(&c->s, s);
     return;
   }
-  c->stat_items += s.in_count;
-  c->stat_heavy += s.heavy_count;
-  c->stat_rounds += 1;
+  stat_add(&c->stat_items, s.in_count);
+  stat_add(&c->stat_heavy, s.heavy_count);
+  stat_add(&c->stat_rounds, 1);
+  if (a.trace && s.round - 1 < a.trace_cap) {
+    lsg_round_trace &r = a.trace[s.round - 1];
+    r.round = s.round;
+    r.phase = s.phase;
+    r.in_items = s.in_count;
+    r.heavy_chunks = s.heavy_count;
+    r.near_out = nnear;
+    r.far_out = nfar;
+    r.heavy_out = nheavy;
+    r.curdelta_lo = s.curdelta_lo;
+    r.curdelta_hi = s.curdelta_hi;
+    r.delta = s.delta;
+    r.t_end = wall_clock64();
+  }
   uint32_t consumed = s.in_q;
-  st_relaxed(&c->hcount[s.heavy_in], 0u);  // consumed heavy queue becomes the next producer
+  st_coh(&c->hcount[s.heavy_in], 0u);  // consumed heavy queue becomes the next producer
   s.heavy_in = 1u - s.heavy_in;
   s.heavy_count = nheavy;
   if (nnear > 0 || nheavy > 0) {
     s.in_q = s.near_q;
     s.in_count = nnear;
     s.near_q = consumed;
-    st_relaxed(&c->qcount[consumed], 0u);
+    st_coh(&c->qcount[consumed], 0u);
     s.round += 1;
   } else if (A == Algo::SSSP && nfar > 0) {
     s.in_q = s.far_q;
     s.in_count = nfar;
     s.far_q = consumed;
-    st_relaxed(&c->qcount[consumed], 0u);
-    s.curdelta += s.delta;
+    st_coh(&c->qcount[consumed], 0u);
+    s.set_curdelta(s.curdelta() + s.delta);
     s.phase += 1;
     s.round += 1;
-    c->stat_phases += 1;
+    stat_add(&c->stat_phases, 1);
   } else {
     s.done = 1;
-    c->stat_phases += 1;
+    stat_add(&c->stat_phases, 1);
   }
-  s.ntiles = 0;
-  c->s = s;
-  st_relaxed(&c->ticket, 0u);
+  sched_store(&c->s, s);
+  st_coh(&c->ticket, 0u);
 }
 
 template <Algo A, int BS>
@@ -315,22 +337,24 @@ __global__ __launch_bounds__(BS) void persistent_kernel(KArgs a) {
   __shared__ uint32_t quit;
   Ctrl *c = a.ctrl;
   const uint32_t nb = gridDim.x;
+  if (!residency_handshake(c, nb, a.residency_limit)) return;
   for (;;) {
     if (threadIdx.x == 0) {
-      Sched s = c->s;
-      s.ntiles = (s.in_count + BS - 1) / BS;
+      Sched s;
+      sched_load(s, &c->s);
       ss = s;
-      quit = s.done || ld_relaxed(&c->abort) != kAbortNone;
+      quit = s.done || ld_coh(&c->abort) != kAbortNone;
     }
     __syncthreads();
     if (quit) break;
     const Sched s = ss;
-    const uint32_t total = s.ntiles + s.heavy_count;
+    const uint32_t ntiles = (s.in_count + BS - 1) / BS;
+    const uint32_t total = ntiles + s.heavy_count;
     // Static first pass (ticket = block id), then dynamic tickets.
     uint32_t t = blockIdx.x;
     while (t < total) {
-      if (t < s.ntiles) process_tile<A, BS>(a, s, t, sh);
-      else process_heavy<A, BS>(a, s, t - s.ntiles);
+      if (t < ntiles) process_tile<A, BS>(a, s, t, sh);
+      else process_heavy<A, BS>(a, s, t - ntiles);
       if (total <= nb) break;
       __syncthreads();
       if (threadIdx.x == 0) tk = nb + atomic_add(&c->ticket, 1u);
@@ -361,17 +385,16 @@ __global__ void init_kernel(KArgs a, uint32_t src, uint32_t delta) {
     c->hcount[1] = 0;
     c->ticket = 0;
     c->abort = kAbortNone;
+    c->checkin = 0;
+    c->resident_seen = 0;
     Sched s;
     memset(&s, 0, sizeof s);
     s.in_q = 0;
     s.near_q = 1;
     s.far_q = 2;
     s.in_count = 1;
-    s.heavy_in = 0;
-    s.heavy_count = 0;
     s.round = 1;
-    s.phase = 0;
-    s.curdelta = 0;  // apps/sssp/kernel.cu:472
+    s.set_curdelta(0);  // apps/sssp/kernel.cu:472
     s.delta = delta;
     c->s = s;
     c->stat_items = c->stat_rounds = c->stat_heavy = c->stat_phases = 0;
@@ -397,6 +420,15 @@ struct Workspace {
   uint32_t cap_nodes = 0, cap_heavy = 0;
   uint32_t *q = nullptr, *hq = nullptr, *stamps = nullptr;
   Ctrl *ctrl = nullptr;
+  struct GridEntry {
+    const void *fn;
+    int bs, grid;
+  };
+  std::vector<GridEntry> grids;  // calibrated persistent grid per kernel
+  lsg_round_trace *trace = nullptr;  // device trace buffer (allocated on first traced call)
+  uint32_t trace_cap = 0;
+  std::vector<lsg_round_trace> last_trace;  // host copy of the last traced call
+  uint64_t trace_t0 = 0;
   std::mutex mu;
 };
 
@@ -463,6 +495,31 @@ static uint64_t spin_limit_ticks(const Workspace *w) {
   return (uint64_t)(sec * w->wall_khz * 1000.0);
 }
 
+// Start-up handshake bound: blocks of one launch begin within microseconds,
+// so 50 ms means the missing blocks are waiting for a slot.
+static uint64_t residency_limit_ticks(const Workspace *w) { return (uint64_t)(0.05 * w->wall_khz * 1000.0); }
+
+// Largest grid believed co-resident for (kernel, block size): the occupancy
+// query times the CU count, lowered to what a launch actually observed.
+static int resident_grid(Workspace *w, const void *fn, int bs, int *grid) {
+  for (auto &e : w->grids)
+    if (e.fn == fn && e.bs == bs) {
+      *grid = e.grid;
+      return LSG_OK;
+    }
+  int per_cu = 0;
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, bs, 0));
+  if (per_cu <= 0) return fail(LSG_ERR_HIP, "persistent kernel cannot be resident (block %d)", bs);
+  *grid = per_cu * w->cus;
+  w->grids.push_back({fn, bs, *grid});
+  return LSG_OK;
+}
+
+static void record_resident_grid(Workspace *w, const void *fn, int bs, int grid) {
+  for (auto &e : w->grids)
+    if (e.fn == fn && e.bs == bs) e.grid = grid;
+}
+
 // Runs one traversal into dist (device array of g->nnodes).  Synchronous.
 template <Algo A>
 static int run_single(const lsg_graph *g, const lsg_opts *o_in, uint32_t *dist, lsg_stats *st) {
@@ -515,28 +572,59 @@ static int run_single(const lsg_graph *g, const lsg_opts *o_in, uint32_t *dist, 
   a.far_mode = (uint32_t)o.far_mode;
   a.spin_limit = spin_limit_ticks(w);
 
-  KernelFn fn = A == Algo::SSSP ? pick_kernel<Algo::SSSP>(bs) : pick_kernel<Algo::BFS>(bs);
-  int per_cu = 0;
-  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)fn, bs, 0));
-  if (per_cu <= 0) return fail(LSG_ERR_HIP, "persistent kernel cannot be resident (block %d)", bs);
-  int max_grid = per_cu * w->cus;
-  int grid = o.grid_blocks > 0 ? std::min(o.grid_blocks, max_grid) : max_grid;
+  a.residency_limit = residency_limit_ticks(w);
+  if (o.trace) {
+    if (!w->trace) {
+      w->trace_cap = 1u << 16;
+      if (hipMalloc(&w->trace, (size_t)w->trace_cap * sizeof(lsg_round_trace)) != hipSuccess) {
+        w->trace = nullptr;
+        w->trace_cap = 0;
+        return fail(LSG_ERR_NOMEM, "trace buffer: out of device memory");
+      }
+    }
+    a.trace = w->trace;
+    a.trace_cap = w->trace_cap;
+  }
 
-  HIPCHK(hipEventRecord(w->ev0, w->stream));
-  uint32_t init_blocks = std::min<uint32_t>((g->nnodes + 255) / 256, 8u * w->cus);
-  hipLaunchKernelGGL(init_kernel, dim3(init_blocks), dim3(256), 0, w->stream, a, (uint32_t)o.start_node, a.delta);
-  HIPCHK(hipGetLastError());
-  void *args[] = {&a};
-  HIPCHK(hipLaunchCooperativeKernel((const void *)fn, dim3(grid), dim3(bs), args, 0, w->stream));
-  HIPCHK(hipEventRecord(w->ev1, w->stream));
+  KernelFn fn = A == Algo::SSSP ? pick_kernel<Algo::SSSP>(bs) : pick_kernel<Algo::BFS>(bs);
+  int grid = 0;
+  rc = resident_grid(w, (const void *)fn, bs, &grid);
+  if (rc) return rc;
+  if (o.grid_blocks > 0) grid = std::min(o.grid_blocks, grid);
+
   Ctrl hc;
-  HIPCHK(hipMemcpyAsync(&hc, w->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, w->stream));
-  HIPCHK(hipStreamSynchronize(w->stream));
   float ms = 0;
-  HIPCHK(hipEventElapsedTime(&ms, w->ev0, w->ev1));
+  for (int attempt = 0;; ++attempt) {
+    HIPCHK(hipEventRecord(w->ev0, w->stream));
+    uint32_t init_blocks = std::min<uint32_t>((g->nnodes + 255) / 256, 8u * w->cus);
+    hipLaunchKernelGGL(init_kernel, dim3(init_blocks), dim3(256), 0, w->stream, a, (uint32_t)o.start_node, a.delta);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(bs), 0, w->stream, a);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(w->ev1, w->stream));
+    HIPCHK(hipMemcpyAsync(&hc, w->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, w->stream));
+    HIPCHK(hipStreamSynchronize(w->stream));
+    HIPCHK(hipEventElapsedTime(&ms, w->ev0, w->ev1));
+    if (hc.abort != kAbortResidency) break;
+    // Fewer blocks were resident than the occupancy query promised: remember
+    // the count that was, and run again on that grid.
+    if (attempt > 0 || hc.resident_seen == 0 || hc.resident_seen >= (uint32_t)grid)
+      return fail(LSG_ERR_HIP, "persistent grid of %d x %d blocks is not co-resident (%u were)", grid, bs,
+                  hc.resident_seen);
+    grid = (int)hc.resident_seen;
+    record_resident_grid(w, (const void *)fn, bs, grid);
+  }
   if (hc.abort == kAbortTimeout)
-    return fail(LSG_ERR_TIMEOUT, "grid barrier timed out (grid %d x %d); distances are incomplete", grid, bs);
+    return fail(LSG_ERR_TIMEOUT,
+                "grid barrier timed out (grid %d x %d) at round %u phase %u done %u with %u of %d blocks "
+                "arrived, generation %u; distances are incomplete",
+                grid, bs, hc.s.round, hc.s.phase, hc.s.done, hc.bar_count, grid, hc.bar_gen);
   if (hc.abort == kAbortCapacity) return fail(LSG_ERR_CAPACITY, "device queue capacity exceeded");
+  if (o.trace) {
+    uint32_t n = (uint32_t)std::min<uint64_t>(hc.stat_rounds, w->trace_cap);
+    w->last_trace.resize(n);
+    if (n) HIPCHK(hipMemcpy(w->last_trace.data(), w->trace, n * sizeof(lsg_round_trace), hipMemcpyDeviceToHost));
+  }
   if (st) {
     memset(st, 0, sizeof *st);
     st->ms_kernel = ms;
@@ -544,7 +632,7 @@ static int run_single(const lsg_graph *g, const lsg_opts *o_in, uint32_t *dist, 
     st->near_iters = hc.stat_rounds;
     st->items = hc.stat_items;
     st->heavy_chunks = hc.stat_heavy;
-    st->last_delta = (int32_t)std::min<int64_t>(hc.s.curdelta, INT_MAX);
+    st->last_delta = (int32_t)std::min<int64_t>(hc.s.curdelta(), INT_MAX);
     st->grid_blocks = grid;
     st->block_size = bs;
     st->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -659,6 +747,18 @@ int lsg_sssp_dist(const lsg_graph *gg, const lsg_opts *opts, int32_t *dist_dev, 
 int lsg_bfs_dist(const lsg_graph *gg, const lsg_opts *opts, int32_t *level_dev, lsg_stats *stats) {
   clear_error();
   return run_single<Algo::BFS>(gg, opts, (uint32_t *)level_dev, stats);
+}
+
+int lsg_last_trace(int device, lsg_round_trace *out, uint32_t cap, uint32_t *n, double *tick_hz) {
+  Workspace *w;
+  int rc = get_workspace(device, &w);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(w->mu);
+  uint32_t m = (uint32_t)std::min<size_t>(cap, w->last_trace.size());
+  if (out && m) memcpy(out, w->last_trace.data(), m * sizeof(lsg_round_trace));
+  if (n) *n = (uint32_t)w->last_trace.size();
+  if (tick_hz) *tick_hz = w->wall_khz * 1000.0;
+  return LSG_OK;
 }
 
 int lsg_sssp_gg_main(lsg_graph *hg, lsg_graph *gg, const lsg_opts *opts, lsg_stats *stats) {

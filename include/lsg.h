@@ -82,8 +82,24 @@ typedef struct lsg_opts {
   int32_t thread_deg;  /* TWB: degrees below this are expanded by one lane (default 8) */
   int32_t heavy_deg;   /* degrees >= this are split into chunks of heavy_deg edges (default 4096) */
   int32_t sched;       /* 0 = fixed delta, 1 = learned per-phase delta (lsg_sched_set_model) */
-  int32_t reserved[6];
+  int32_t trace;       /* 1 = record one lsg_round_trace per round (lsg_last_trace) */
+  int32_t reserved[5];
 } lsg_opts;
+
+/* One record per round of a single-source traversal, written when
+ * lsg_opts.trace is set.  No reference counterpart: it is the input of the
+ * learned scheduler's trace generator and of the autotuner. */
+typedef struct lsg_round_trace {
+  uint32_t round, phase;
+  uint32_t in_items;     /* frontier items popped this round */
+  uint32_t heavy_chunks; /* heavy chunks consumed this round */
+  uint32_t near_out;     /* items pushed to the near pile */
+  uint32_t far_out;      /* far pile size after this round */
+  uint32_t heavy_out;    /* heavy chunks queued for the next round */
+  uint32_t curdelta_lo, curdelta_hi;
+  uint32_t delta;        /* width in force this round */
+  uint64_t t_end;        /* device wall clock at the end of the round */
+} lsg_round_trace;
 
 /* Run statistics.  ms_total mirrors the reference's timed region around
  * gg_main (skelapp/skel.cu:60-63), which includes worklist allocation
@@ -162,6 +178,9 @@ int lsg_bfs_gg_main(lsg_graph *hg, lsg_graph *gg, const lsg_opts *opts, lsg_stat
  * several result arrays resident). */
 int lsg_sssp_dist(const lsg_graph *gg, const lsg_opts *opts, int32_t *dist_dev, lsg_stats *stats);
 int lsg_bfs_dist(const lsg_graph *gg, const lsg_opts *opts, int32_t *level_dev, lsg_stats *stats);
+/* Copies the round records of the last traced call on `device` into out[cap];
+ * *n gets the number of records, *tick_hz the device wall-clock rate. */
+int lsg_last_trace(int device, lsg_round_trace *out, uint32_t cap, uint32_t *n, double *tick_hz);
 
 /* output() of apps/sssp and apps/bfs (apps/sssp/support.cu:34-53): one line
  * per node, "%d INF\n" for INT_MAX and "%d %d\n" otherwise.  path NULL writes
