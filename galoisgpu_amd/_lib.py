@@ -9,7 +9,7 @@ from __future__ import annotations
 
 import ctypes
 import os
-from ctypes import POINTER, c_char_p, c_double, c_int, c_int32, c_size_t, c_uint32, c_uint64, c_void_p
+from ctypes import POINTER, c_char_p, c_double, c_float, c_int, c_int32, c_size_t, c_uint32, c_uint64, c_void_p
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liblsg.so")
@@ -114,9 +114,12 @@ class LsgBatchStats(ctypes.Structure):
         ("phases", c_uint64),
         ("items", c_uint64),
         ("heavy_chunks", c_uint64),
+        ("lane_items", c_uint64),
+        ("edges_expanded", c_uint64),
         ("grid_blocks", c_int32),
         ("block_size", c_int32),
-        ("reserved", c_int32 * 6),
+        ("launches", c_int32),
+        ("reserved", c_int32 * 5),
     ]
 
     def as_dict(self) -> dict:
@@ -155,6 +158,19 @@ SIGNATURES = {
     "lsg_last_trace": (c_int, [c_int, POINTER(LsgRoundTrace), c_uint32, POINTER(c_uint32), POINTER(c_double)]),
     "lsg_output": (c_int, [POINTER(LsgGraph), c_char_p]),
     "lsg_output_dist": (c_int, [c_void_p, c_uint32, c_char_p]),
+    "lsg_sssp_multi": (c_int, [POINTER(LsgGraph), c_void_p, c_uint32, POINTER(LsgOpts), c_void_p,
+                               POINTER(LsgBatchStats)]),
+    "lsg_sssp_batch": (c_int, [POINTER(LsgGraph), c_void_p, c_uint32, POINTER(LsgOpts), c_void_p,
+                               POINTER(LsgBatchStats)]),
+    "lsg_last_features": (c_int, [c_int, c_void_p, c_uint32, POINTER(c_uint32)]),
+    "lsg_sched_model_floats": (c_int, []),
+    "lsg_sched_set_model": (c_int, [c_int, c_void_p, c_uint32]),
+    "lsg_mlp_train": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_uint32, c_uint32, c_float,
+                              c_float, c_uint64, c_void_p]),
+    "lsg_mlp_infer": (c_int, [c_int, c_void_p, c_void_p, c_uint32, c_void_p]),
+    "lsg_dev_malloc": (c_int, [c_int, c_size_t, POINTER(c_void_p)]),
+    "lsg_dev_free": (c_int, [c_int, c_void_p]),
+    "lsg_dev_copy": (c_int, [c_int, c_void_p, c_void_p, c_size_t, c_int]),
     "lsg_set_device": (c_int, [c_int]),
     "lsg_device_name": (c_int, [c_int, c_char_p, c_size_t]),
     "lsg_device_sync": (c_int, [c_int]),

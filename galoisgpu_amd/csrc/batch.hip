@@ -1,0 +1,624 @@
+// Batched SSSP on gfx950: up to 64 sources on one graph in one persistent
+// launch, vertex-major.
+//
+// No reference counterpart: the reference runs one source per process
+// (apps/sssp/kernel.cu:455-480).  Each source follows exactly the plain
+// method's rules -- fixed-delta near/far piles, curdelta += DELTA per phase
+// (apps/sssp/kernel.cu:129-144,355-391) -- so its distances are identical to a
+// single-source run; what changes is that all sources share every round.
+//
+// Layout in HBM: D[v][s] (u32, row stride S <= 64): the 64 lanes of a
+// wavefront carry the 64 sources, so relaxing edge u->v for every source is one
+// coalesced row load of D[v] and masked atomicMin on the same row, and the
+// edge's column and weight are read once for all sources.  Per-vertex 64-bit
+// masks say which sources must expand a vertex: nmask[2][V] for the near pile
+// (indexed by round parity: written in round r, consumed in round r+1) and
+// fmask[2][V] for the far pile (phase parity).  A vertex enters a pile's queue
+// when its mask goes from 0 to non-zero, so each queue holds a vertex once.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cstring>
+#include <vector>
+
+#include "engine.h"
+#include "lsg_internal.h"
+#include "runtime.h"
+#include "sched_mlp.h"
+
+namespace lsg {
+
+struct MArgs {
+  const uint32_t *row_start;
+  const uint32_t *edge_dst;
+  const uint32_t *edge_data;
+  uint32_t *D;          // [V][S]
+  uint64_t *nmask[2];   // near masks by round parity
+  uint64_t *fmask[2];   // far masks by phase parity
+  uint32_t *q[3];       // item queues (vertex ids)
+  uint64_t *hq[2];      // heavy chunks: two u64 words each: (k << 32 | u), mask
+  const uint32_t *srcs; // S source vertices
+  Ctrl *ctrl;
+  uint32_t nnodes, S, qcap, hcap, delta, heavy_deg;
+  uint64_t spin_limit, residency_limit;
+  lsg_round_trace *trace;
+  uint32_t trace_cap;
+  uint32_t sched;       // 1 = learned delta per phase
+  const float *mlp;     // scheduler weights (sched_mlp.h layout)
+  uint64_t nedges;
+  uint32_t far_mode;    // initial far-pile mode (0 reference, 1 split)
+  float *feat;          // per-phase feature records [kMaxForced][kFeatRec], or null
+};
+
+// Counters the leader turns into trace records and scheduler features.
+struct MultiCounters {
+  uint64_t lane_items;   // sum over expanded items of active sources
+  uint64_t edges;        // edges expanded (per vertex, shared by its sources)
+  uint64_t improved;     // rows in which at least one lane improved
+};
+
+__device__ __forceinline__ uint64_t bcast64(uint64_t x, uint32_t l) {
+  uint32_t lo = readlane((uint32_t)x, l), hi = readlane((uint32_t)(x >> 32), l);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Relaxes edges [e_begin, e_end) of u for the sources whose lanes hold a
+// finite du.  Row loads of four edges are issued before any is used.
+__device__ __forceinline__ void expand_edges(const MArgs &a, const Sched &s, uint32_t du, bool act,
+                                             uint32_t e_begin, uint32_t e_end, uint64_t *improved_rows) {
+  const uint32_t lane = lane_id();
+  const int64_t curdelta = s.curdelta();
+  uint64_t *nm = a.nmask[s.round & 1];
+  uint64_t *fm = a.fmask[s.phase & 1];
+  uint32_t *qn = a.q[s.near_q], *qf = a.q[s.far_q];
+  for (uint32_t b = e_begin; b < e_end; b += kWave) {
+    uint32_t e = b + lane;
+    uint32_t myv = 0, myw = 0;
+    if (e < e_end) {
+      myv = a.edge_dst[e];
+      myw = a.edge_data[e];
+    }
+    const uint32_t n = min((uint32_t)kWave, e_end - b);
+    bool pn = false, pf = false;
+    for (uint32_t j0 = 0; j0 < n; j0 += 4) {
+      uint32_t v[4], nd[4], cur[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        uint32_t j = min(j0 + k, n - 1);
+        v[k] = readlane(myv, j);
+        nd[k] = du + readlane(myw, j);
+        cur[k] = act ? ld_coh(a.D + (size_t)v[k] * a.S + lane) : 0u;
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        if (j0 + k >= n) break;
+        bool want = act && nd[k] < cur[k];
+        if (__ballot(want) == 0) continue;
+        uint32_t old = want ? atomic_min(a.D + (size_t)v[k] * a.S + lane, nd[k]) : 0u;
+        bool imp = want && nd[k] < old;
+        bool nearl = imp && (int64_t)nd[k] <= curdelta;
+        uint64_t nb = __ballot(nearl), fb = __ballot(imp && !nearl);
+        if (nb | fb) *improved_rows += 1;
+        if (lane == j0 + k) {
+          if (nb) pn = atomic_or64(nm + v[k], nb) == 0;
+          if (fb) pf = atomic_or64(fm + v[k], fb) == 0;
+        }
+      }
+    }
+    wave_push(pn, myv, qn, &a.ctrl->qcount[s.near_q], a.qcap, &a.ctrl->abort);
+    wave_push(pf, myv, qf, &a.ctrl->qcount[s.far_q], a.qcap, &a.ctrl->abort);
+  }
+}
+
+// One frontier item: consume its mask, load its distance row, and expand it
+// (or queue it as heavy chunks for the next round).
+__device__ __forceinline__ void process_item(const MArgs &a, const Sched &s, uint32_t i, MultiCounters &mc) {
+  const uint32_t lane = lane_id();
+  uint32_t u = ld_coh(a.q[s.in_q] + i);
+  if (u >= a.nnodes) return;
+  uint64_t *mp = s.aux[0] ? a.fmask[(s.phase - 1) & 1] + u : a.nmask[(s.round - 1) & 1] + u;
+  uint64_t m = 0;
+  if (lane == 0) m = atomic_exch64(mp, 0ull);
+  m = bcast64(m, 0);
+  if (m == 0) return;
+  uint32_t rs = a.row_start[u], re = a.row_start[u + 1];
+  if (re - rs >= a.heavy_deg) {
+    uint32_t nch = (re - rs + a.heavy_deg - 1) / a.heavy_deg;
+    uint32_t hqi = 1u - s.heavy_in;
+    uint32_t base = 0;
+    if (lane == 0) base = atomic_add(&a.ctrl->hcount[hqi], nch);
+    base = readlane(base, 0);
+    for (uint32_t k = lane; k < nch; k += kWave) {
+      uint32_t pos = base + k;
+      if (pos < a.hcap) {
+        st_coh64(a.hq[hqi] + 2ull * pos, ((uint64_t)k << 32) | u);
+        st_coh64(a.hq[hqi] + 2ull * pos + 1, m);
+      } else {
+        st_coh(&a.ctrl->abort, kAbortCapacity);
+      }
+    }
+    return;
+  }
+  bool act = lane < a.S && ((m >> lane) & 1);
+  uint32_t du = act ? ld_coh(a.D + (size_t)u * a.S + lane) : kInf;
+  act = act && du != kInf;
+  if (s.aux[0] && s.aux[1]) {
+    // Split far pile: a source whose distance is still above this phase's
+    // threshold waits for a later phase instead of expanding early.
+    bool defer = act && (int64_t)du > s.curdelta();
+    uint64_t dm = __ballot(defer);
+    if (dm) {
+      bool push = false;
+      if (lane == 0) push = atomic_or64(a.fmask[s.phase & 1] + u, dm) == 0;
+      wave_push(push, u, a.q[s.far_q], &a.ctrl->qcount[s.far_q], a.qcap, &a.ctrl->abort);
+      act = act && !defer;
+    }
+  }
+  uint64_t am = __ballot(act);
+  if (am == 0) return;
+  mc.lane_items += __builtin_popcountll(am);
+  mc.edges += re - rs;
+  expand_edges(a, s, du, act, rs, re, &mc.improved);
+}
+
+__device__ __forceinline__ void process_chunk(const MArgs &a, const Sched &s, uint32_t c, MultiCounters &mc) {
+  const uint32_t lane = lane_id();
+  uint64_t ent = ld_coh64(a.hq[s.heavy_in] + 2ull * c);
+  uint64_t m = ld_coh64(a.hq[s.heavy_in] + 2ull * c + 1);
+  uint32_t u = (uint32_t)ent, k = (uint32_t)(ent >> 32);
+  if (u >= a.nnodes) return;
+  uint32_t rs = a.row_start[u], re = a.row_start[u + 1];
+  if ((uint64_t)k * a.heavy_deg >= re - rs) return;
+  uint32_t lo = rs + k * a.heavy_deg, hi = min(re, lo + a.heavy_deg);
+  bool act = lane < a.S && ((m >> lane) & 1);
+  uint32_t du = act ? ld_coh(a.D + (size_t)u * a.S + lane) : kInf;
+  act = act && du != kInf;
+  uint64_t am = __ballot(act);
+  if (am == 0) return;
+  mc.lane_items += __builtin_popcountll(am);
+  mc.edges += hi - lo;
+  expand_edges(a, s, du, act, lo, hi, &mc.improved);
+}
+
+__device__ __forceinline__ void stat_add64(uint64_t *p, uint64_t v) { st_coh64(p, ld_coh64(p) + v); }
+
+// Leader step, run by every thread of the last block to reach the barrier:
+// the pile rotation of apps/sssp/kernel.cu:363-384 for all sources at once,
+// plus, at each phase boundary, the learned scheduler's choice of the next
+// phase's delta width and far-pile mode.  Schedule words: aux[0] = the
+// in-queue is the far pile, aux[1] = split far mode for this phase.
+__device__ void multi_leader(const MArgs &a, MultiCounters *tot) {
+  __shared__ Sched s;
+  __shared__ SchedFeatures f;
+  __shared__ MlpScratch sc;
+  __shared__ uint32_t phase_end, consumed;
+  Ctrl *c = a.ctrl;
+  const uint32_t t = threadIdx.x;
+  if (t == 0) {
+    sched_load(s, &c->s);
+    uint32_t nnear = ld_coh(&c->qcount[s.near_q]);
+    uint32_t nfar = ld_coh(&c->qcount[s.far_q]);
+    uint32_t nheavy = ld_coh(&c->hcount[1u - s.heavy_in]);
+    phase_end = 0;
+    if (nnear > a.qcap || nfar > a.qcap || nheavy > a.hcap) {
+      st_coh(&c->abort, kAbortCapacity);
+      s.done = 1;
+    } else {
+      stat_add64(&c->stat_items, s.in_count);
+      stat_add64(&c->stat_heavy, s.heavy_count);
+      stat_add64(&c->stat_rounds, 1);
+      if (a.trace && s.round - 1 < a.trace_cap) {
+        lsg_round_trace &r = a.trace[s.round - 1];
+        r.round = s.round;
+        r.phase = s.phase;
+        r.in_items = s.in_count;
+        r.heavy_chunks = s.heavy_count;
+        r.near_out = nnear;
+        r.far_out = nfar;
+        r.heavy_out = nheavy;
+        r.curdelta_lo = s.curdelta_lo;
+        r.curdelta_hi = s.curdelta_hi;
+        r.delta = s.delta;
+        r.t_end = wall_clock64();
+      }
+      consumed = s.in_q;
+      st_coh(&c->hcount[s.heavy_in], 0u);
+      s.heavy_in = 1u - s.heavy_in;
+      s.heavy_count = nheavy;
+      if (nnear > 0 || nheavy > 0) {
+        s.in_q = s.near_q;
+        s.in_count = nnear;
+        s.near_q = consumed;
+        st_coh(&c->qcount[consumed], 0u);
+        s.aux[0] = 0;
+        s.round += 1;
+      } else if (nfar > 0) {
+        phase_end = 1;
+        f.in_items = nfar;
+        f.phase = s.phase;
+        f.round = s.round;
+        f.curdelta = (float)s.curdelta();
+        f.base_delta = a.delta;
+        f.prev_delta = s.delta;
+        f.lane_items = (float)ld_coh64(&tot->lane_items);
+        f.edges = (float)ld_coh64(&tot->edges);
+        f.nnodes = a.nnodes;
+        f.nedges = (float)a.nedges;
+        f.nsrc = a.S;
+      } else {
+        s.done = 1;
+        stat_add64(&c->stat_phases, 1);
+      }
+    }
+  }
+  __syncthreads();
+  if (!phase_end) {
+    if (t == 0) sched_store(&c->s, s);
+    return;
+  }
+  int act = -1;
+  if (a.sched) act = mlp_block_action(a.mlp, f, sc);
+  if (t == 0) {
+    if (act >= 0) {
+      s.delta = action_delta(a.mlp, act, a.delta);
+      s.aux[1] = a.mlp[kOffFarMode + act] > 0.5f ? 1u : 0u;
+    }
+    if (a.feat && s.phase < (uint32_t)kMaxForced) {
+      float *rec = a.feat + (size_t)s.phase * kFeatRec;
+      sched_featurize(f, rec);
+      rec[kFeat] = (float)act;
+      rec[kFeat + 1] = (float)s.delta;
+      rec[kFeat + 2] = (float)s.aux[1];
+      rec[kFeat + 3] = (float)(s.round + 1);  // first round of the phase being started
+    }
+    s.in_q = s.far_q;
+    s.in_count = f.in_items;
+    s.far_q = consumed;
+    st_coh(&c->qcount[consumed], 0u);
+    s.aux[0] = 1;
+    s.set_curdelta(s.curdelta() + s.delta);
+    s.phase += 1;
+    s.round += 1;
+    stat_add64(&c->stat_phases, 1);
+    sched_store(&c->s, s);
+  }
+}
+
+template <int BS>
+__global__ __launch_bounds__(BS) void multi_kernel(MArgs a, MultiCounters *tot) {
+  __shared__ Sched ss;
+  __shared__ uint32_t quit;
+  __shared__ unsigned long long blk[3];
+  Ctrl *c = a.ctrl;
+  const uint32_t nb = gridDim.x;
+  if (!residency_handshake(c, nb, a.residency_limit)) return;
+  const uint32_t wpb = BS / kWave;
+  const uint32_t wave = blockIdx.x * wpb + threadIdx.x / kWave;
+  const uint32_t nwaves = nb * wpb;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      Sched s;
+      sched_load(s, &c->s);
+      ss = s;
+      quit = s.done || ld_coh(&c->abort) != kAbortNone;
+      blk[0] = blk[1] = blk[2] = 0;
+    }
+    __syncthreads();
+    if (quit) break;
+    const Sched s = ss;
+    MultiCounters mc = {0, 0, 0};
+    for (uint32_t i = wave; i < s.in_count; i += nwaves) process_item(a, s, i, mc);
+    for (uint32_t k = wave; k < s.heavy_count; k += nwaves) process_chunk(a, s, k, mc);
+    if (lane_id() == 0 && (mc.lane_items | mc.edges)) {
+      atomicAdd(&blk[0], (unsigned long long)mc.lane_items);
+      atomicAdd(&blk[1], (unsigned long long)mc.edges);
+      atomicAdd(&blk[2], (unsigned long long)mc.improved);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && (blk[0] | blk[1])) {
+      __hip_atomic_fetch_add(&tot->lane_items, (uint64_t)blk[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(&tot->edges, (uint64_t)blk[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(&tot->improved, (uint64_t)blk[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    grid_barrier_block(c, nb, a.spin_limit, [&] { multi_leader(a, tot); });
+  }
+}
+
+// D[v][s] = INF, masks cleared.
+__global__ void multi_init(MArgs a, MultiCounters *tot) {
+  size_t tid = blockIdx.x * (size_t)blockDim.x + threadIdx.x, nt = (size_t)gridDim.x * blockDim.x;
+  size_t cells = (size_t)a.nnodes * a.S;
+  for (size_t i = tid; i < cells; i += nt) a.D[i] = kInf;
+  for (size_t v = tid; v < a.nnodes; v += nt) {
+    a.nmask[0][v] = 0;
+    a.nmask[1][v] = 0;
+    a.fmask[0][v] = 0;
+    a.fmask[1][v] = 0;
+  }
+  if (tid == 0) *tot = MultiCounters{0, 0, 0};
+}
+
+// Seeds every source (apps/sssp/kernel.cu:33,476-477) and the control block.
+// One wavefront; lane s seeds source s.
+__global__ void multi_seed(MArgs a) {
+  const uint32_t lane = threadIdx.x;
+  bool act = lane < a.S;
+  uint32_t src = act ? a.srcs[lane] : 0u;
+  bool push = false;
+  if (act) {
+    a.D[(size_t)src * a.S + lane] = 0u;
+    push = atomic_or64(a.nmask[0] + src, 1ull << lane) == 0;
+  }
+  uint64_t pm = __ballot(push);
+  if (push) a.q[0][lanes_below(pm)] = src;
+  if (lane == 0) {
+    Ctrl *c = a.ctrl;
+    c->bar_count = 0;
+    c->bar_gen = 0;
+    c->qcount[0] = (uint32_t)__builtin_popcountll(pm);
+    c->qcount[1] = 0;
+    c->qcount[2] = 0;
+    c->hcount[0] = 0;
+    c->hcount[1] = 0;
+    c->ticket = 0;
+    c->abort = kAbortNone;
+    c->checkin = 0;
+    c->resident_seen = 0;
+    Sched s;
+    memset(&s, 0, sizeof s);
+    s.in_q = 0;
+    s.near_q = 1;
+    s.far_q = 2;
+    s.in_count = (uint32_t)__builtin_popcountll(pm);
+    s.round = 1;
+    s.set_curdelta(0);
+    s.delta = a.delta;
+    s.aux[1] = a.far_mode;
+    c->s = s;
+    c->stat_items = c->stat_rounds = c->stat_heavy = c->stat_phases = 0;
+  }
+}
+
+// out[k][v] = D[v][k] for the units of one launch (per-source arrays).
+__global__ void multi_scatter(const uint32_t *D, uint32_t S, uint32_t nnodes, int32_t *const *outs) {
+  size_t tid = blockIdx.x * (size_t)blockDim.x + threadIdx.x, nt = (size_t)gridDim.x * blockDim.x;
+  size_t cells = (size_t)nnodes * S;
+  for (size_t i = tid; i < cells; i += nt) {
+    uint32_t v = (uint32_t)(i / S), k = (uint32_t)(i % S);
+    outs[k][v] = (int32_t)D[i];
+  }
+}
+
+using MultiFn = void (*)(MArgs, MultiCounters *);
+
+static MultiFn pick_multi(int bs) {
+  switch (bs) {
+    case 256: return multi_kernel<256>;
+    case 1024: return multi_kernel<1024>;
+    default: return multi_kernel<512>;
+  }
+}
+
+struct MultiRun {
+  double ms_kernel = 0;
+  uint64_t rounds = 0, phases = 0, items = 0, heavy = 0, lane_items = 0, edges = 0;
+  int grid = 0, bs = 0;
+};
+
+// Runs one vertex-major launch: S = nsrc <= 64 sources of `g` into D [V][S].
+static int run_multi(Workspace *w, const lsg_graph *g, const uint32_t *srcs_host, uint32_t nsrc, const lsg_opts &o,
+                     uint32_t *D, MultiRun *out) {
+  int rc;
+  int bs = o.block_size ? o.block_size : 512;
+  if (bs != 256 && bs != 512 && bs != 1024) return fail(LSG_ERR_ARG, "block_size must be 256, 512 or 1024");
+  uint32_t heavy_deg = o.heavy_deg > 0 ? (uint32_t)o.heavy_deg : 2048u;
+  uint32_t V = g->nnodes;
+  uint64_t hcap64 = 2ull * g->nedges / heavy_deg + 64;
+  uint32_t hcap = (uint32_t)std::min<uint64_t>(hcap64, UINT32_MAX / 4);
+  void *qbuf, *hbuf, *mbuf, *sbuf;
+  if ((rc = ws_buffer(w, kBufQueues, 3ull * V * 4, &qbuf)) || (rc = ws_buffer(w, kBufHeavy, 2ull * hcap * 16, &hbuf)) ||
+      (rc = ws_buffer(w, kBufMasks, 4ull * V * 8 + 256, &mbuf)) || (rc = ws_buffer(w, kBufSeeds, 64 * 4 + 256, &sbuf)))
+    return rc;
+  MArgs a;
+  memset(&a, 0, sizeof a);
+  a.row_start = g->row_start;
+  a.edge_dst = g->edge_dst;
+  a.edge_data = (const uint32_t *)g->edge_data;
+  a.D = D;
+  uint64_t *masks = (uint64_t *)mbuf;
+  a.nmask[0] = masks;
+  a.nmask[1] = masks + V;
+  a.fmask[0] = masks + 2ull * V;
+  a.fmask[1] = masks + 3ull * V;
+  for (int i = 0; i < 3; ++i) a.q[i] = (uint32_t *)qbuf + (size_t)i * V;
+  a.hq[0] = (uint64_t *)hbuf;
+  a.hq[1] = (uint64_t *)hbuf + 2ull * hcap;
+  a.srcs = (const uint32_t *)sbuf;
+  a.ctrl = w->ctrl;
+  a.nnodes = V;
+  a.S = nsrc;
+  a.qcap = V;
+  a.hcap = hcap;
+  a.delta = (uint32_t)o.delta;
+  a.heavy_deg = heavy_deg;
+  a.spin_limit = spin_limit_ticks(w);
+  a.residency_limit = residency_limit_ticks(w);
+  a.nedges = g->nedges;
+  MultiCounters *tot = (MultiCounters *)((char *)sbuf + 64 * 4);
+  if (o.sched) {
+    a.sched = 1;
+    a.mlp = sched_model_device(w->device);
+    if (!a.mlp) return fail(LSG_ERR_ARG, "learned scheduler requested but no model is loaded (lsg_sched_set_model)");
+  }
+  if (o.far_mode != LSG_FAR_REFERENCE && o.far_mode != LSG_FAR_SPLIT)
+    return fail(LSG_ERR_ARG, "bad far_mode %d", o.far_mode);
+  a.far_mode = (uint32_t)o.far_mode;
+  if (o.trace) {
+    void *tbuf;
+    size_t tbytes = (1u << 16) * sizeof(lsg_round_trace);
+    if ((rc = ws_buffer(w, kBufTrace, tbytes + kMaxForced * kFeatRec * sizeof(float), &tbuf))) return rc;
+    a.trace = (lsg_round_trace *)tbuf;
+    a.trace_cap = 1u << 16;
+    a.feat = (float *)((char *)tbuf + tbytes);
+  }
+  HIPCHK(hipMemcpyAsync(sbuf, srcs_host, nsrc * 4, hipMemcpyHostToDevice, w->stream));
+  MultiFn fn = pick_multi(bs);
+  int grid = 0;
+  if ((rc = resident_grid(w, (const void *)fn, bs, &grid))) return rc;
+  if (o.grid_blocks > 0) grid = std::min(o.grid_blocks, grid);
+  Ctrl hc;
+  float ms = 0;
+  for (int attempt = 0;; ++attempt) {
+    HIPCHK(hipEventRecord(w->ev0, w->stream));
+    uint32_t init_blocks = (uint32_t)std::min<uint64_t>(((uint64_t)V * nsrc + 255) / 256, 16ull * w->cus);
+    hipLaunchKernelGGL(multi_init, dim3(init_blocks), dim3(256), 0, w->stream, a, tot);
+    hipLaunchKernelGGL(multi_seed, dim3(1), dim3(64), 0, w->stream, a);
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(bs), 0, w->stream, a, tot);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(w->ev1, w->stream));
+    HIPCHK(hipMemcpyAsync(&hc, w->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, w->stream));
+    HIPCHK(hipStreamSynchronize(w->stream));
+    HIPCHK(hipEventElapsedTime(&ms, w->ev0, w->ev1));
+    if (hc.abort != kAbortResidency) break;
+    if (attempt > 0 || hc.resident_seen == 0 || hc.resident_seen >= (uint32_t)grid)
+      return fail(LSG_ERR_HIP, "persistent grid of %d x %d blocks is not co-resident (%u were)", grid, bs,
+                  hc.resident_seen);
+    grid = (int)hc.resident_seen;
+    record_resident_grid(w, (const void *)fn, bs, grid);
+  }
+  if (hc.abort == kAbortTimeout)
+    return fail(LSG_ERR_TIMEOUT, "grid barrier timed out (grid %d x %d) at round %u; distances are incomplete",
+                grid, bs, hc.s.round);
+  if (hc.abort == kAbortCapacity) return fail(LSG_ERR_CAPACITY, "device queue capacity exceeded");
+  MultiCounters ht;
+  HIPCHK(hipMemcpy(&ht, tot, sizeof ht, hipMemcpyDeviceToHost));
+  if (o.trace) {
+    uint32_t n = (uint32_t)std::min<uint64_t>(hc.stat_rounds, a.trace_cap);
+    w->last_trace.resize(n);
+    if (n) HIPCHK(hipMemcpy(w->last_trace.data(), a.trace, n * sizeof(lsg_round_trace), hipMemcpyDeviceToHost));
+    // one record per phase boundary: phase p -> p + 1 is record p
+    uint32_t np = std::min<uint32_t>(hc.s.phase, kMaxForced);
+    w->last_feat.resize((size_t)np * kFeatRec);
+    if (np) HIPCHK(hipMemcpy(w->last_feat.data(), a.feat, w->last_feat.size() * sizeof(float), hipMemcpyDeviceToHost));
+  }
+  out->ms_kernel = ms;
+  out->rounds = hc.stat_rounds;
+  out->phases = hc.stat_phases;
+  out->items = hc.stat_items;
+  out->heavy = hc.stat_heavy;
+  out->lane_items = ht.lane_items;
+  out->edges = ht.edges;
+  out->grid = grid;
+  out->bs = bs;
+  return LSG_OK;
+}
+
+static int check_multi_args(const lsg_graph *g, const uint32_t *srcs, uint32_t nsrc, const lsg_opts &o) {
+  if (!g || !g->device_graph) return fail(LSG_ERR_ARG, "expected a device graph (copy_to_gpu first)");
+  if (!srcs || nsrc == 0) return fail(LSG_ERR_ARG, "no sources");
+  if (o.delta <= 0) return fail(LSG_ERR_ARG, "delta must be > 0 (got %d)", o.delta);
+  for (uint32_t i = 0; i < nsrc; ++i)
+    if (srcs[i] >= g->nnodes) return fail(LSG_ERR_ARG, "source %u = %u out of range [0, %u)", i, srcs[i], g->nnodes);
+  return LSG_OK;
+}
+
+static void fill_stats(lsg_batch_stats *st, const MultiRun &r, int launches) {
+  st->ms_kernel += r.ms_kernel;
+  st->rounds += r.rounds;
+  st->phases += r.phases;
+  st->items += r.items;
+  st->heavy_chunks += r.heavy;
+  st->lane_items += r.lane_items;
+  st->edges_expanded += r.edges;
+  st->grid_blocks = r.grid;
+  st->block_size = r.bs;
+  st->launches = launches;
+}
+
+}  // namespace lsg
+
+using namespace lsg;
+
+extern "C" {
+
+int lsg_sssp_multi(const lsg_graph *g, const uint32_t *srcs, uint32_t nsrc, const lsg_opts *opts, int32_t *dist_vs,
+                   lsg_batch_stats *stats) {
+  clear_error();
+  auto t0 = std::chrono::steady_clock::now();
+  lsg_opts o;
+  if (opts) o = *opts;
+  else lsg_opts_default(&o);
+  int rc = check_multi_args(g, srcs, nsrc, o);
+  if (rc) return rc;
+  if (nsrc > 64) return fail(LSG_ERR_ARG, "lsg_sssp_multi takes at most 64 sources (got %u)", nsrc);
+  if (!dist_vs) return fail(LSG_ERR_ARG, "NULL distance matrix");
+  Workspace *w;
+  if ((rc = get_workspace(g->device, &w))) return rc;
+  std::lock_guard<std::mutex> lk(w->mu);
+  HIPCHK(hipSetDevice(w->device));
+  MultiRun r;
+  if ((rc = run_multi(w, g, srcs, nsrc, o, (uint32_t *)dist_vs, &r))) return rc;
+  if (stats) {
+    memset(stats, 0, sizeof *stats);
+    fill_stats(stats, r, 1);
+    stats->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  return LSG_OK;
+}
+
+int lsg_last_features(int device, float *out, uint32_t cap_phases, uint32_t *nphases) {
+  Workspace *w;
+  int rc = get_workspace(device, &w);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(w->mu);
+  uint32_t np = (uint32_t)(w->last_feat.size() / kFeatRec);
+  uint32_t m = std::min(np, cap_phases);
+  if (out && m) memcpy(out, w->last_feat.data(), (size_t)m * kFeatRec * sizeof(float));
+  if (nphases) *nphases = np;
+  return LSG_OK;
+}
+
+int lsg_sssp_batch(const lsg_graph *g, const uint32_t *srcs, uint32_t nsrc, const lsg_opts *opts,
+                   int32_t *const *dist_out, lsg_batch_stats *stats) {
+  clear_error();
+  auto t0 = std::chrono::steady_clock::now();
+  lsg_opts o;
+  if (opts) o = *opts;
+  else lsg_opts_default(&o);
+  int rc = check_multi_args(g, srcs, nsrc, o);
+  if (rc) return rc;
+  if (!dist_out) return fail(LSG_ERR_ARG, "NULL output array list");
+  for (uint32_t i = 0; i < nsrc; ++i)
+    if (!dist_out[i]) return fail(LSG_ERR_ARG, "output %u is NULL", i);
+  Workspace *w;
+  if ((rc = get_workspace(g->device, &w))) return rc;
+  std::lock_guard<std::mutex> lk(w->mu);
+  HIPCHK(hipSetDevice(w->device));
+  uint32_t S = std::min<uint32_t>(nsrc, 64);
+  void *dbuf, *pbuf;
+  if ((rc = ws_buffer(w, kBufTranspose, (size_t)g->nnodes * S * 4 + 64 * sizeof(void *), &dbuf))) return rc;
+  pbuf = (char *)dbuf + (size_t)g->nnodes * S * 4;
+  if (stats) memset(stats, 0, sizeof *stats);
+  int launches = 0;
+  for (uint32_t base = 0; base < nsrc; base += 64) {
+    uint32_t n = std::min<uint32_t>(64, nsrc - base);
+    MultiRun r;
+    if ((rc = run_multi(w, g, srcs + base, n, o, (uint32_t *)dbuf, &r))) return rc;
+    ++launches;
+    HIPCHK(hipMemcpyAsync(pbuf, dist_out + base, n * sizeof(void *), hipMemcpyHostToDevice, w->stream));
+    uint32_t blocks = (uint32_t)std::min<uint64_t>(((uint64_t)g->nnodes * n + 255) / 256, 16ull * w->cus);
+    hipLaunchKernelGGL(multi_scatter, dim3(blocks), dim3(256), 0, w->stream, (const uint32_t *)dbuf, n, g->nnodes,
+                       (int32_t *const *)pbuf);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(w->stream));
+    if (stats) fill_stats(stats, r, launches);
+  }
+  if (stats)
+    stats->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return LSG_OK;
+}
+
+}  // extern "C"

@@ -206,6 +206,47 @@ __device__ __forceinline__ void grid_barrier(Ctrl *c, uint32_t nblocks, uint64_t
   __syncthreads();
 }
 
+// Same barrier, but every thread of the last-arriving block runs `leader()`
+// (which may use __syncthreads and LDS), so leader work such as scheduler
+// inference can be spread over the block.
+template <class Leader>
+__device__ __forceinline__ void grid_barrier_block(Ctrl *c, uint32_t nblocks, uint64_t spin_limit, Leader leader) {
+  __shared__ uint32_t s_lead, s_gen;
+  order_release();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t gen = ld_coh(&c->bar_gen);
+    uint32_t arrived = atomic_add(&c->bar_count, 1u);
+    s_lead = arrived == nblocks - 1;
+    s_gen = gen;
+  }
+  __syncthreads();
+  if (s_lead) {
+    order_acquire();
+    leader();
+    order_release();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      st_coh(&c->bar_count, 0u);
+      order_release();
+      st_coh(&c->bar_gen, s_gen + 1);
+    }
+  } else if (threadIdx.x == 0) {
+    uint32_t gen = s_gen;
+    uint64_t t0 = wall_clock64();
+    while (ld_coh(&c->bar_gen) == gen) {
+      if (ld_coh(&c->abort) != kAbortNone) break;
+      if (wall_clock64() - t0 > spin_limit) {
+        st_coh(&c->abort, kAbortTimeout);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+  }
+  order_acquire();
+  __syncthreads();
+}
+
 // Residency handshake, run once at kernel start.  A persistent grid is only
 // correct if every block is resident at once; the runtime's occupancy figure
 // has proven optimistic, so each block checks in and waits (bounded by

@@ -26,10 +26,13 @@
 
 #include "engine.h"
 #include "lsg_internal.h"
+#include "runtime.h"
 
 namespace lsg {
 
 enum class Algo : int { SSSP = 0, BFS = 1 };
+
+constexpr uint32_t kTraceCap = 1u << 16;  // round records kept per traced call
 
 struct KArgs {
   const uint32_t *row_start;
@@ -404,79 +407,6 @@ __global__ void init_kernel(KArgs a, uint32_t src, uint32_t delta) {
 
 // ---- host side ------------------------------------------------------------
 
-#define HIPCHK(expr)                                                                        \
-  do {                                                                                      \
-    hipError_t _e = (expr);                                                                 \
-    if (_e != hipSuccess) return fail(LSG_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(_e)); \
-  } while (0)
-
-// Per-device workspace, grown on demand and kept for later calls.
-struct Workspace {
-  int device = -1;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  int cus = 0;
-  double wall_khz = 0;
-  uint32_t cap_nodes = 0, cap_heavy = 0;
-  uint32_t *q = nullptr, *hq = nullptr, *stamps = nullptr;
-  Ctrl *ctrl = nullptr;
-  struct GridEntry {
-    const void *fn;
-    int bs, grid;
-  };
-  std::vector<GridEntry> grids;  // calibrated persistent grid per kernel
-  lsg_round_trace *trace = nullptr;  // device trace buffer (allocated on first traced call)
-  uint32_t trace_cap = 0;
-  std::vector<lsg_round_trace> last_trace;  // host copy of the last traced call
-  uint64_t trace_t0 = 0;
-  std::mutex mu;
-};
-
-static std::mutex g_ws_mu;
-static std::vector<Workspace *> g_ws;
-
-static int get_workspace(int device, Workspace **out) {
-  std::lock_guard<std::mutex> lk(g_ws_mu);
-  for (auto *w : g_ws)
-    if (w->device == device) {
-      *out = w;
-      return LSG_OK;
-    }
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(LSG_ERR_NODEV, "no HIP device available");
-  if (device < 0 || device >= ndev) return fail(LSG_ERR_ARG, "device %d out of range (%d devices)", device, ndev);
-  HIPCHK(hipSetDevice(device));
-  auto *w = new Workspace;
-  w->device = device;
-  HIPCHK(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking));
-  HIPCHK(hipEventCreate(&w->ev0));
-  HIPCHK(hipEventCreate(&w->ev1));
-  HIPCHK(hipDeviceGetAttribute(&w->cus, hipDeviceAttributeMultiprocessorCount, device));
-  int khz = 0;
-  HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device));
-  w->wall_khz = khz > 0 ? khz : 100000.0;
-  HIPCHK(hipMalloc(&w->ctrl, sizeof(Ctrl)));
-  g_ws.push_back(w);
-  *out = w;
-  return LSG_OK;
-}
-
-static int ensure_capacity(Workspace *w, uint32_t nnodes, uint32_t heavy) {
-  if (nnodes <= w->cap_nodes && heavy <= w->cap_heavy) return LSG_OK;
-  uint32_t n = std::max(nnodes, w->cap_nodes), h = std::max(heavy, w->cap_heavy);
-  if (w->q) HIPCHK(hipFree(w->q));
-  if (w->hq) HIPCHK(hipFree(w->hq));
-  if (w->stamps) HIPCHK(hipFree(w->stamps));
-  w->q = w->hq = w->stamps = nullptr;
-  w->cap_nodes = w->cap_heavy = 0;
-  if (hipMalloc(&w->q, 3ull * n * 4) != hipSuccess || hipMalloc(&w->hq, 2ull * 2 * h * 4) != hipSuccess ||
-      hipMalloc(&w->stamps, 2ull * n * 4) != hipSuccess)
-    return fail(LSG_ERR_NOMEM, "device workspace for %u nodes: out of memory", n);
-  w->cap_nodes = n;
-  w->cap_heavy = h;
-  return LSG_OK;
-}
-
 using KernelFn = void (*)(KArgs);
 
 template <Algo A>
@@ -486,38 +416,6 @@ static KernelFn pick_kernel(int bs) {
     case 1024: return persistent_kernel<A, 1024>;
     default: return persistent_kernel<A, 256>;
   }
-}
-
-static uint64_t spin_limit_ticks(const Workspace *w) {
-  double sec = 20.0;
-  if (const char *e = getenv("LSG_BARRIER_TIMEOUT_S")) sec = atof(e);
-  if (sec < 0) sec = 0;
-  return (uint64_t)(sec * w->wall_khz * 1000.0);
-}
-
-// Start-up handshake bound: blocks of one launch begin within microseconds,
-// so 50 ms means the missing blocks are waiting for a slot.
-static uint64_t residency_limit_ticks(const Workspace *w) { return (uint64_t)(0.05 * w->wall_khz * 1000.0); }
-
-// Largest grid believed co-resident for (kernel, block size): the occupancy
-// query times the CU count, lowered to what a launch actually observed.
-static int resident_grid(Workspace *w, const void *fn, int bs, int *grid) {
-  for (auto &e : w->grids)
-    if (e.fn == fn && e.bs == bs) {
-      *grid = e.grid;
-      return LSG_OK;
-    }
-  int per_cu = 0;
-  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, bs, 0));
-  if (per_cu <= 0) return fail(LSG_ERR_HIP, "persistent kernel cannot be resident (block %d)", bs);
-  *grid = per_cu * w->cus;
-  w->grids.push_back({fn, bs, *grid});
-  return LSG_OK;
-}
-
-static void record_resident_grid(Workspace *w, const void *fn, int bs, int grid) {
-  for (auto &e : w->grids)
-    if (e.fn == fn && e.bs == bs) e.grid = grid;
 }
 
 // Runs one traversal into dist (device array of g->nnodes).  Synchronous.
@@ -547,8 +445,11 @@ static int run_single(const lsg_graph *g, const lsg_opts *o_in, uint32_t *dist, 
   HIPCHK(hipSetDevice(w->device));
   uint64_t hcap64 = 2ull * g->nedges / heavy_deg + 64;
   uint32_t hcap = (uint32_t)std::min<uint64_t>(hcap64, UINT32_MAX / 2);
-  rc = ensure_capacity(w, g->nnodes, hcap);
-  if (rc) return rc;
+  void *qbuf, *hbuf, *sbuf;
+  if ((rc = ws_buffer(w, kBufQueues, 3ull * g->nnodes * 4, &qbuf)) ||
+      (rc = ws_buffer(w, kBufHeavy, 2ull * hcap * 8, &hbuf)) ||
+      (rc = ws_buffer(w, kBufStamps, 2ull * g->nnodes * 4, &sbuf)))
+    return rc;
 
   KArgs a;
   memset(&a, 0, sizeof a);
@@ -556,15 +457,15 @@ static int run_single(const lsg_graph *g, const lsg_opts *o_in, uint32_t *dist, 
   a.edge_dst = g->edge_dst;
   a.edge_data = (const uint32_t *)g->edge_data;
   a.dist = dist;
-  for (int i = 0; i < 3; ++i) a.q[i] = w->q + (size_t)i * w->cap_nodes;
-  a.hq[0] = w->hq;
-  a.hq[1] = w->hq + 2ull * w->cap_heavy;
-  a.near_stamp = w->stamps;
-  a.far_stamp = w->stamps + w->cap_nodes;
+  for (int i = 0; i < 3; ++i) a.q[i] = (uint32_t *)qbuf + (size_t)i * g->nnodes;
+  a.hq[0] = (uint32_t *)hbuf;
+  a.hq[1] = (uint32_t *)hbuf + 2ull * hcap;
+  a.near_stamp = (uint32_t *)sbuf;
+  a.far_stamp = (uint32_t *)sbuf + g->nnodes;
   a.ctrl = w->ctrl;
   a.nnodes = g->nnodes;
-  a.qcap = w->cap_nodes;
-  a.hcap = w->cap_heavy;
+  a.qcap = g->nnodes;
+  a.hcap = hcap;
   a.delta = A == Algo::SSSP ? (uint32_t)o.delta : 1u;
   a.thread_deg = thread_deg;
   a.heavy_deg = heavy_deg;
@@ -574,16 +475,10 @@ static int run_single(const lsg_graph *g, const lsg_opts *o_in, uint32_t *dist, 
 
   a.residency_limit = residency_limit_ticks(w);
   if (o.trace) {
-    if (!w->trace) {
-      w->trace_cap = 1u << 16;
-      if (hipMalloc(&w->trace, (size_t)w->trace_cap * sizeof(lsg_round_trace)) != hipSuccess) {
-        w->trace = nullptr;
-        w->trace_cap = 0;
-        return fail(LSG_ERR_NOMEM, "trace buffer: out of device memory");
-      }
-    }
-    a.trace = w->trace;
-    a.trace_cap = w->trace_cap;
+    void *tbuf;
+    if ((rc = ws_buffer(w, kBufTrace, kTraceCap * sizeof(lsg_round_trace), &tbuf))) return rc;
+    a.trace = (lsg_round_trace *)tbuf;
+    a.trace_cap = kTraceCap;
   }
 
   KernelFn fn = A == Algo::SSSP ? pick_kernel<Algo::SSSP>(bs) : pick_kernel<Algo::BFS>(bs);
@@ -621,9 +516,9 @@ static int run_single(const lsg_graph *g, const lsg_opts *o_in, uint32_t *dist, 
                 grid, bs, hc.s.round, hc.s.phase, hc.s.done, hc.bar_count, grid, hc.bar_gen);
   if (hc.abort == kAbortCapacity) return fail(LSG_ERR_CAPACITY, "device queue capacity exceeded");
   if (o.trace) {
-    uint32_t n = (uint32_t)std::min<uint64_t>(hc.stat_rounds, w->trace_cap);
+    uint32_t n = (uint32_t)std::min<uint64_t>(hc.stat_rounds, kTraceCap);
     w->last_trace.resize(n);
-    if (n) HIPCHK(hipMemcpy(w->last_trace.data(), w->trace, n * sizeof(lsg_round_trace), hipMemcpyDeviceToHost));
+    if (n) HIPCHK(hipMemcpy(w->last_trace.data(), a.trace, n * sizeof(lsg_round_trace), hipMemcpyDeviceToHost));
   }
   if (st) {
     memset(st, 0, sizeof *st);

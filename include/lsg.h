@@ -182,12 +182,60 @@ int lsg_bfs_dist(const lsg_graph *gg, const lsg_opts *opts, int32_t *level_dev, 
  * *n gets the number of records, *tick_hz the device wall-clock rate. */
 int lsg_last_trace(int device, lsg_round_trace *out, uint32_t cap, uint32_t *n, double *tick_hz);
 
+/* ---- batched SSSP (no reference counterpart) ------------------------------ */
+typedef struct lsg_batch_stats {
+  double ms_total;
+  double ms_kernel;        /* device time of all launches (HIP events) */
+  uint64_t rounds, phases, items, heavy_chunks;
+  uint64_t lane_items;     /* (vertex, source) expansions */
+  uint64_t edges_expanded; /* edge expansions, each shared by the sources of its vertex */
+  int32_t grid_blocks, block_size;
+  int32_t launches;        /* persistent launches (one per 64 sources) */
+  int32_t reserved[5];
+} lsg_batch_stats;
+
+/* Up to 64 sources of one device graph in ONE persistent launch.  dist_vs is a
+ * device array [nnodes][nsrc] (row v holds the distances of v from every
+ * source, in source order).  Each column equals lsg_sssp_dist with the same
+ * opts.delta bit for bit.  srcs is a host array. */
+int lsg_sssp_multi(const lsg_graph *g, const uint32_t *srcs, uint32_t nsrc, const lsg_opts *opts, int32_t *dist_vs,
+                   lsg_batch_stats *stats);
+/* Any number of sources of one device graph, 64 per launch; dist_out[i] is a
+ * device array of nnodes int32 for source i (host array of device pointers). */
+int lsg_sssp_batch(const lsg_graph *g, const uint32_t *srcs, uint32_t nsrc, const lsg_opts *opts,
+                   int32_t *const *dist_out, lsg_batch_stats *stats);
+/* Per-phase feature records of the last traced batched call: nphases rows of
+ * 20 floats (16 scheduler features, action or -1, delta, far mode, first round). */
+int lsg_last_features(int device, float *out, uint32_t cap_phases, uint32_t *nphases);
+
+/* ---- learned scheduler (no reference counterpart) ------------------------- */
+/* Size in floats of a scheduler model (normalisation, 16-64-64-8 MLP weights,
+ * per-action delta multiplier and far mode, forced per-phase actions). */
+int lsg_sched_model_floats(void);
+/* Loads the model used when lsg_opts.sched = 1 on `device`. */
+int lsg_sched_set_model(int device, const float *model, uint32_t nfloats);
+/* Trains the MLP part of `model` in place on n samples (X: n x 16 raw
+ * features, y: action labels, weights: per-sample loss weights or NULL) with
+ * minibatch SGD + momentum in a HIP kernel; also sets the feature
+ * normalisation.  *loss gets the mean loss over the last 100 steps. */
+int lsg_mlp_train(int device, float *model, const float *X, const int32_t *y, const float *weights, uint32_t n,
+                  uint32_t steps, float lr, float momentum, uint64_t seed, float *loss);
+/* Argmax actions for n feature rows (HIP kernel, one thread per row). */
+int lsg_mlp_infer(int device, const float *model, const float *X, uint32_t n, int32_t *labels);
+
 /* output() of apps/sssp and apps/bfs (apps/sssp/support.cu:34-53): one line
  * per node, "%d INF\n" for INT_MAX and "%d %d\n" otherwise.  path NULL writes
  * nothing; "-" writes stdout.  `g` must be a host graph. */
 int lsg_output(const lsg_graph *g, const char *path);
 /* The same text for an explicit distance array. */
 int lsg_output_dist(const int32_t *dist, uint32_t n, const char *path);
+
+/* Device memory for callers without a GPU runtime of their own (the Python
+ * host side, tests, benchmarks).  kind: 0 host->device, 1 device->host,
+ * 2 device->device. */
+int lsg_dev_malloc(int device, size_t bytes, void **ptr);
+int lsg_dev_free(int device, void *ptr);
+int lsg_dev_copy(int device, void *dst, const void *src, size_t bytes, int kind);
 
 /* Device synchronisation and name, for the CLI's "Using GPU:" line
  * (skelapp/skel.cu:146-148). */

@@ -1,0 +1,131 @@
+"""GPU parity of the batched (multi-source) path and the learned-scheduler kernels.
+
+Every column of a batched run must equal the oracle's distances bit for bit,
+whatever delta width, far-pile mode or forced per-phase schedule is used.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from galoisgpu_amd import CSRGraph, Options
+from galoisgpu_amd import sched as lsched
+from galoisgpu_amd.batch import last_features, sssp_batch, sssp_multi
+
+pytestmark = pytest.mark.gpu
+INF = 0x7FFFFFFF
+
+
+def say(*a):
+    print("[gpu-test]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def oracle_cols(g, srcs):
+    return np.stack([oracle.dijkstra(g.row_start, g.edge_dst, g.edge_data, int(s)) for s in srcs], axis=1)
+
+
+def test_multi_small_golden(small_graphs):
+    for name, gg in small_graphs.items():
+        g = CSRGraph.from_arrays(gg["row"], gg["col"], gg["w"])
+        n = g.nnodes
+        srcs = [gg["src"]] + [int(x) for x in np.random.default_rng(1).integers(0, n, 7)]
+        ref = oracle_cols(g, srcs)
+        assert np.array_equal(ref[:, 0], gg["dist"])
+        dev = g.copy_to_gpu()
+        for delta in (1, 5, 32):
+            for far in (0, 1):
+                say("multi small", name, delta, far)
+                d, _ = sssp_multi(dev, srcs, Options(delta=delta, far_mode=far))
+                assert np.array_equal(d, ref), (name, delta, far)
+        dev.close()
+
+
+@pytest.fixture(scope="module")
+def rmat16():
+    g = CSRGraph.rmat(16, 16, seed=3)
+    srcs = np.random.default_rng(5).integers(0, g.nnodes, 64)
+    srcs[10] = srcs[3]  # a duplicated source
+    srcs[0] = g.max_degree_vertex()
+    return g, srcs, oracle_cols(g, srcs)
+
+
+@pytest.mark.parametrize("opts", [dict(), dict(far_mode=1), dict(block_size=256, heavy_deg=256),
+                                  dict(block_size=1024, delta=7), dict(delta=10000)])
+def test_multi_rmat16(rmat16, opts):
+    g, srcs, ref = rmat16
+    dev = g.copy_to_gpu()
+    o = dict(delta=32)
+    o.update(opts)
+    say("multi rmat16", o)
+    d, st = sssp_multi(dev, srcs, Options(**o))
+    assert np.array_equal(d, ref), o
+    assert st["launches"] == 1 and st["rounds"] > 0
+    dev.close()
+
+
+def test_multi_partial_width_and_batch(rmat16):
+    g, srcs, ref = rmat16
+    dev = g.copy_to_gpu()
+    for S in (1, 5, 32, 63):
+        say("multi width", S)
+        d, _ = sssp_multi(dev, srcs[:S], Options(delta=32))
+        assert np.array_equal(d, ref[:, :S]), S
+    many = np.concatenate([srcs, srcs[:37]])
+    say("batch 101 sources")
+    outs, st = sssp_batch(dev, many, Options(delta=32))
+    assert st["launches"] == 2
+    for k, s in enumerate(many):
+        assert np.array_equal(outs[k], ref[:, k % 64]), k
+    dev.close()
+
+
+def test_forced_schedules_keep_distances(rmat16):
+    """Distances are identical whatever schedule the scheduler imposes."""
+    g, srcs, ref = rmat16
+    dev = g.copy_to_gpu()
+    base = lsched.new_model(0)
+    rng = np.random.default_rng(9)
+    schedules = [[a] for a in range(lsched.N_ACT)] + [list(rng.integers(0, lsched.N_ACT, 40)) for _ in range(4)]
+    for sch in schedules:
+        lsched.set_model(lsched.forced(base, sch))
+        say("forced", sch[:6])
+        d, st = sssp_multi(dev, srcs, Options(delta=32, sched=1, trace=True))
+        assert np.array_equal(d, ref), sch
+        feats = last_features()
+        assert len(feats) == st["phases"] - 1
+        acts = feats[:, 16].astype(int)
+        assert all(acts[p] == sch[min(p, len(sch) - 1)] for p in range(len(acts)))
+    # an untrained (random) network as the scheduler: still exact
+    lsched.set_model(base)
+    d, _ = sssp_multi(dev, srcs, Options(delta=32, sched=1))
+    assert np.array_equal(d, ref)
+    dev.close()
+
+
+def numpy_forward(m, X):
+    o = lsched
+    x = (X - m[o.OFF_MEAN:o.OFF_MEAN + 16]) * m[o.OFF_INVSTD:o.OFF_INVSTD + 16]
+    W1 = m[o.OFF_W1:o.OFF_B1].reshape(16, 64)
+    W2 = m[o.OFF_W2:o.OFF_B2].reshape(64, 64)
+    W3 = m[o.OFF_W3:o.OFF_B3].reshape(64, 8)
+    h1 = np.maximum(x @ W1 + m[o.OFF_B1:o.OFF_B1 + 64], 0)
+    h2 = np.maximum(h1 @ W2 + m[o.OFF_B2:o.OFF_B2 + 64], 0)
+    return h2 @ W3 + m[o.OFF_B3:o.OFF_B3 + 8]
+
+
+def test_mlp_train_and_infer_kernels():
+    """The HIP trainer learns a separable rule; HIP inference matches an fp32 numpy forward."""
+    rng = np.random.default_rng(3)
+    X = rng.normal(0, 1, (3000, 16)).astype(np.float32) * 3 + 1
+    y = ((X[:, 0] > 1).astype(int) + 2 * (X[:, 3] > 1).astype(int) + 4 * (X[:, 7] > 1).astype(int)).astype(np.int32)
+    say("mlp train")
+    m, loss = lsched.train(X, y, steps=3000, lr=0.05)
+    pred = lsched.infer(m, X)
+    acc = float((pred == y).mean())
+    assert acc > 0.95 and loss < 0.3, (acc, loss)
+    logits = numpy_forward(m.astype(np.float64), X.astype(np.float64))
+    top2 = np.sort(logits, axis=1)[:, -2:]
+    clear = (top2[:, 1] - top2[:, 0]) > 1e-3  # skip near-ties, where fp32 order may differ
+    assert np.array_equal(pred[clear], np.argmax(logits, axis=1)[clear])
