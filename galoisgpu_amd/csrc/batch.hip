@@ -31,6 +31,15 @@
 
 namespace lsg {
 
+#ifndef LSG_MULTI_UNROLL
+#define LSG_MULTI_UNROLL 4
+#endif
+#ifndef LSG_MULTI_WAVES
+#define LSG_MULTI_WAVES 0
+#endif
+// Row loads issued ahead per wavefront in the edge loop.
+constexpr int kUnroll = LSG_MULTI_UNROLL;
+
 struct MArgs {
   const uint32_t *row_start;
   const uint32_t *edge_dst;
@@ -49,8 +58,17 @@ struct MArgs {
   uint32_t sched;       // 1 = learned delta per phase
   const float *mlp;     // scheduler weights (sched_mlp.h layout)
   uint64_t nedges;
-  uint32_t far_mode;    // initial far-pile mode (0 reference, 1 split)
+  uint32_t far_mode;    // initial far-pile mode (LSG_FAR_*)
   float *feat;          // per-phase feature records [kMaxForced][kFeatRec], or null
+  uint32_t *fmin;       // [2][64] smallest pending far distance per source, by phase parity
+  uint32_t *thr;        // [64] per-source near/far threshold of the current phase (bucket mode)
+};
+
+// Per-round, per-lane view of the schedule used by the edge loop.
+struct LaneSched {
+  uint32_t thr;   // near/far threshold of this lane's source
+  bool bucket;    // per-source thresholds (LSG_FAR_BUCKET phase)
+  uint32_t fmin;  // smallest distance this lane pushed to the far pile this round
 };
 
 // Counters the leader turns into trace records and scheduler features.
@@ -67,10 +85,12 @@ __device__ __forceinline__ uint64_t bcast64(uint64_t x, uint32_t l) {
 
 // Relaxes edges [e_begin, e_end) of u for the sources whose lanes hold a
 // finite du.  Row loads of four edges are issued before any is used.
-__device__ __forceinline__ void expand_edges(const MArgs &a, const Sched &s, uint32_t du, bool act,
+__device__ __forceinline__ void expand_edges(const MArgs &a, const Sched &s, LaneSched &ls, uint32_t du, bool act,
                                              uint32_t e_begin, uint32_t e_end, uint64_t *improved_rows) {
   const uint32_t lane = lane_id();
-  const int64_t curdelta = s.curdelta();
+  // Near/far threshold of this lane: the shared curdelta (apps/sssp/kernel.cu:134)
+  // or, in a bucket phase, the lane's own source threshold.
+  const int64_t thr = ls.bucket ? (int64_t)ls.thr : s.curdelta();
   uint64_t *nm = a.nmask[s.round & 1];
   uint64_t *fm = a.fmask[s.phase & 1];
   uint32_t *qn = a.q[s.near_q], *qf = a.q[s.far_q];
@@ -83,23 +103,24 @@ __device__ __forceinline__ void expand_edges(const MArgs &a, const Sched &s, uin
     }
     const uint32_t n = min((uint32_t)kWave, e_end - b);
     bool pn = false, pf = false;
-    for (uint32_t j0 = 0; j0 < n; j0 += 4) {
-      uint32_t v[4], nd[4], cur[4];
+    for (uint32_t j0 = 0; j0 < n; j0 += kUnroll) {
+      uint32_t v[kUnroll], nd[kUnroll], cur[kUnroll];
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
+      for (int k = 0; k < kUnroll; ++k) {
         uint32_t j = min(j0 + k, n - 1);
         v[k] = readlane(myv, j);
         nd[k] = du + readlane(myw, j);
         cur[k] = act ? ld_coh(a.D + (size_t)v[k] * a.S + lane) : 0u;
       }
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
+      for (int k = 0; k < kUnroll; ++k) {
         if (j0 + k >= n) break;
         bool want = act && nd[k] < cur[k];
         if (__ballot(want) == 0) continue;
         uint32_t old = want ? atomic_min(a.D + (size_t)v[k] * a.S + lane, nd[k]) : 0u;
         bool imp = want && nd[k] < old;
-        bool nearl = imp && (int64_t)nd[k] <= curdelta;
+        bool nearl = imp && (int64_t)nd[k] <= thr;
+        if (imp && !nearl) ls.fmin = min(ls.fmin, nd[k]);
         uint64_t nb = __ballot(nearl), fb = __ballot(imp && !nearl);
         if (nb | fb) *improved_rows += 1;
         if (lane == j0 + k) {
@@ -115,7 +136,8 @@ __device__ __forceinline__ void expand_edges(const MArgs &a, const Sched &s, uin
 
 // One frontier item: consume its mask, load its distance row, and expand it
 // (or queue it as heavy chunks for the next round).
-__device__ __forceinline__ void process_item(const MArgs &a, const Sched &s, uint32_t i, MultiCounters &mc) {
+__device__ __forceinline__ void process_item(const MArgs &a, const Sched &s, LaneSched &ls, uint32_t i,
+                                             MultiCounters &mc) {
   const uint32_t lane = lane_id();
   uint32_t u = ld_coh(a.q[s.in_q] + i);
   if (u >= a.nnodes) return;
@@ -124,8 +146,29 @@ __device__ __forceinline__ void process_item(const MArgs &a, const Sched &s, uin
   if (lane == 0) m = atomic_exch64(mp, 0ull);
   m = bcast64(m, 0);
   if (m == 0) return;
+  bool act = lane < a.S && ((m >> lane) & 1);
+  uint32_t du = act ? ld_coh(a.D + (size_t)u * a.S + lane) : kInf;
+  act = act && du != kInf;
+  if (s.aux[0] && s.aux[1]) {
+    // Split / bucket far pile: a source whose distance is still above this
+    // phase's threshold (shared, or its own in a bucket phase) waits for a
+    // later phase instead of expanding early.
+    bool defer = act && (ls.bucket ? du > ls.thr : (int64_t)du > s.curdelta());
+    uint64_t dm = __ballot(defer);
+    if (dm) {
+      bool push = false;
+      if (lane == 0) push = atomic_or64(a.fmask[s.phase & 1] + u, dm) == 0;
+      wave_push(push, u, a.q[s.far_q], &a.ctrl->qcount[s.far_q], a.qcap, &a.ctrl->abort);
+      if (defer) ls.fmin = min(ls.fmin, du);
+      act = act && !defer;
+    }
+  }
+  uint64_t am = __ballot(act);
+  if (am == 0) return;
   uint32_t rs = a.row_start[u], re = a.row_start[u + 1];
   if (re - rs >= a.heavy_deg) {
+    // Heavy vertex: chunks of heavy_deg edges, each expanded by a wavefront
+    // in the next round with the sources active now.
     uint32_t nch = (re - rs + a.heavy_deg - 1) / a.heavy_deg;
     uint32_t hqi = 1u - s.heavy_in;
     uint32_t base = 0;
@@ -135,36 +178,20 @@ __device__ __forceinline__ void process_item(const MArgs &a, const Sched &s, uin
       uint32_t pos = base + k;
       if (pos < a.hcap) {
         st_coh64(a.hq[hqi] + 2ull * pos, ((uint64_t)k << 32) | u);
-        st_coh64(a.hq[hqi] + 2ull * pos + 1, m);
+        st_coh64(a.hq[hqi] + 2ull * pos + 1, am);
       } else {
         st_coh(&a.ctrl->abort, kAbortCapacity);
       }
     }
     return;
   }
-  bool act = lane < a.S && ((m >> lane) & 1);
-  uint32_t du = act ? ld_coh(a.D + (size_t)u * a.S + lane) : kInf;
-  act = act && du != kInf;
-  if (s.aux[0] && s.aux[1]) {
-    // Split far pile: a source whose distance is still above this phase's
-    // threshold waits for a later phase instead of expanding early.
-    bool defer = act && (int64_t)du > s.curdelta();
-    uint64_t dm = __ballot(defer);
-    if (dm) {
-      bool push = false;
-      if (lane == 0) push = atomic_or64(a.fmask[s.phase & 1] + u, dm) == 0;
-      wave_push(push, u, a.q[s.far_q], &a.ctrl->qcount[s.far_q], a.qcap, &a.ctrl->abort);
-      act = act && !defer;
-    }
-  }
-  uint64_t am = __ballot(act);
-  if (am == 0) return;
   mc.lane_items += __builtin_popcountll(am);
   mc.edges += re - rs;
-  expand_edges(a, s, du, act, rs, re, &mc.improved);
+  expand_edges(a, s, ls, du, act, rs, re, &mc.improved);
 }
 
-__device__ __forceinline__ void process_chunk(const MArgs &a, const Sched &s, uint32_t c, MultiCounters &mc) {
+__device__ __forceinline__ void process_chunk(const MArgs &a, const Sched &s, LaneSched &ls, uint32_t c,
+                                              MultiCounters &mc) {
   const uint32_t lane = lane_id();
   uint64_t ent = ld_coh64(a.hq[s.heavy_in] + 2ull * c);
   uint64_t m = ld_coh64(a.hq[s.heavy_in] + 2ull * c + 1);
@@ -180,7 +207,7 @@ __device__ __forceinline__ void process_chunk(const MArgs &a, const Sched &s, ui
   if (am == 0) return;
   mc.lane_items += __builtin_popcountll(am);
   mc.edges += hi - lo;
-  expand_edges(a, s, du, act, lo, hi, &mc.improved);
+  expand_edges(a, s, ls, du, act, lo, hi, &mc.improved);
 }
 
 __device__ __forceinline__ void stat_add64(uint64_t *p, uint64_t v) { st_coh64(p, ld_coh64(p) + v); }
@@ -189,7 +216,7 @@ __device__ __forceinline__ void stat_add64(uint64_t *p, uint64_t v) { st_coh64(p
 // the pile rotation of apps/sssp/kernel.cu:363-384 for all sources at once,
 // plus, at each phase boundary, the learned scheduler's choice of the next
 // phase's delta width and far-pile mode.  Schedule words: aux[0] = the
-// in-queue is the far pile, aux[1] = split far mode for this phase.
+// in-queue is the far pile, aux[1] = far mode of this phase (LSG_FAR_*).
 __device__ void multi_leader(const MArgs &a, MultiCounters *tot) {
   __shared__ Sched s;
   __shared__ SchedFeatures f;
@@ -259,13 +286,28 @@ __device__ void multi_leader(const MArgs &a, MultiCounters *tot) {
     if (t == 0) sched_store(&c->s, s);
     return;
   }
+  __shared__ int act_sh;
   int act = -1;
   if (a.sched) act = mlp_block_action(a.mlp, f, sc);
   if (t == 0) {
     if (act >= 0) {
       s.delta = action_delta(a.mlp, act, a.delta);
-      s.aux[1] = a.mlp[kOffFarMode + act] > 0.5f ? 1u : 0u;
+      s.aux[1] = (uint32_t)a.mlp[kOffFarMode + act];
     }
+    act_sh = act;
+  }
+  __syncthreads();
+  // Per-source thresholds of the next phase: smallest pending far distance
+  // of each source + delta - 1; the recorded minima of the phase after start
+  // empty.  (Tracked in every phase so any phase can switch to bucket mode.)
+  if (t < a.S) {
+    uint32_t m = ld_coh(a.fmin + (s.phase & 1) * 64 + t);
+    uint64_t th = m == kInf ? (uint64_t)kInf : (uint64_t)m + s.delta - 1;
+    st_coh(a.thr + t, (uint32_t)min(th, (uint64_t)kInf - 1));
+    st_coh(a.fmin + ((s.phase + 1) & 1) * 64 + t, kInf);
+  }
+  if (t == 0) {
+    act = act_sh;
     if (a.feat && s.phase < (uint32_t)kMaxForced) {
       float *rec = a.feat + (size_t)s.phase * kFeatRec;
       sched_featurize(f, rec);
@@ -288,7 +330,11 @@ __device__ void multi_leader(const MArgs &a, MultiCounters *tot) {
 }
 
 template <int BS>
-__global__ __launch_bounds__(BS) void multi_kernel(MArgs a, MultiCounters *tot) {
+__global__ __launch_bounds__(BS)
+#if LSG_MULTI_WAVES
+__attribute__((amdgpu_waves_per_eu(LSG_MULTI_WAVES, LSG_MULTI_WAVES)))
+#endif
+void multi_kernel(MArgs a, MultiCounters *tot) {
   __shared__ Sched ss;
   __shared__ uint32_t quit;
   __shared__ unsigned long long blk[3];
@@ -310,8 +356,14 @@ __global__ __launch_bounds__(BS) void multi_kernel(MArgs a, MultiCounters *tot) 
     if (quit) break;
     const Sched s = ss;
     MultiCounters mc = {0, 0, 0};
-    for (uint32_t i = wave; i < s.in_count; i += nwaves) process_item(a, s, i, mc);
-    for (uint32_t k = wave; k < s.heavy_count; k += nwaves) process_chunk(a, s, k, mc);
+    LaneSched ls;
+    ls.bucket = s.aux[1] == LSG_FAR_BUCKET;
+    ls.thr = ls.bucket && lane_id() < a.S ? ld_coh(a.thr + lane_id()) : 0u;
+    ls.fmin = kInf;
+    for (uint32_t i = wave; i < s.in_count; i += nwaves) process_item(a, s, ls, i, mc);
+    for (uint32_t k = wave; k < s.heavy_count; k += nwaves) process_chunk(a, s, ls, k, mc);
+    // this wavefront's smallest far push per source -> phase minimum
+    if (__ballot(ls.fmin != kInf) && ls.fmin != kInf) atomic_min(a.fmin + (s.phase & 1) * 64 + lane_id(), ls.fmin);
     if (lane_id() == 0 && (mc.lane_items | mc.edges)) {
       atomicAdd(&blk[0], (unsigned long long)mc.lane_items);
       atomicAdd(&blk[1], (unsigned long long)mc.edges);
@@ -378,6 +430,11 @@ __global__ void multi_seed(MArgs a) {
     s.delta = a.delta;
     s.aux[1] = a.far_mode;
     c->s = s;
+    for (int k = 0; k < 64; ++k) {
+      a.fmin[k] = kInf;
+      a.fmin[64 + k] = kInf;
+      a.thr[k] = 0u;  // phase 0: threshold 0, as curdelta starts at 0 (apps/sssp/kernel.cu:472)
+    }
     c->stat_items = c->stat_rounds = c->stat_heavy = c->stat_phases = 0;
   }
 }
@@ -420,7 +477,7 @@ static int run_multi(Workspace *w, const lsg_graph *g, const uint32_t *srcs_host
   uint32_t hcap = (uint32_t)std::min<uint64_t>(hcap64, UINT32_MAX / 4);
   void *qbuf, *hbuf, *mbuf, *sbuf;
   if ((rc = ws_buffer(w, kBufQueues, 3ull * V * 4, &qbuf)) || (rc = ws_buffer(w, kBufHeavy, 2ull * hcap * 16, &hbuf)) ||
-      (rc = ws_buffer(w, kBufMasks, 4ull * V * 8 + 256, &mbuf)) || (rc = ws_buffer(w, kBufSeeds, 64 * 4 + 256, &sbuf)))
+      (rc = ws_buffer(w, kBufMasks, 4ull * V * 8 + 256, &mbuf)) || (rc = ws_buffer(w, kBufSeeds, 2048, &sbuf)))
     return rc;
   MArgs a;
   memset(&a, 0, sizeof a);
@@ -447,13 +504,15 @@ static int run_multi(Workspace *w, const lsg_graph *g, const uint32_t *srcs_host
   a.spin_limit = spin_limit_ticks(w);
   a.residency_limit = residency_limit_ticks(w);
   a.nedges = g->nedges;
-  MultiCounters *tot = (MultiCounters *)((char *)sbuf + 64 * 4);
+  MultiCounters *tot = (MultiCounters *)((char *)sbuf + 256);
+  a.fmin = (uint32_t *)((char *)sbuf + 512);
+  a.thr = (uint32_t *)((char *)sbuf + 1024);
   if (o.sched) {
     a.sched = 1;
     a.mlp = sched_model_device(w->device);
     if (!a.mlp) return fail(LSG_ERR_ARG, "learned scheduler requested but no model is loaded (lsg_sched_set_model)");
   }
-  if (o.far_mode != LSG_FAR_REFERENCE && o.far_mode != LSG_FAR_SPLIT)
+  if (o.far_mode != LSG_FAR_REFERENCE && o.far_mode != LSG_FAR_SPLIT && o.far_mode != LSG_FAR_BUCKET)
     return fail(LSG_ERR_ARG, "bad far_mode %d", o.far_mode);
   a.far_mode = (uint32_t)o.far_mode;
   if (o.trace) {

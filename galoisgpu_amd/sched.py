@@ -2,8 +2,8 @@
 
 The model (layout in csrc/sched_mlp.h) maps 16 frontier features, taken by the
 persistent kernel's barrier leader at every phase boundary, to one of 8
-actions: delta multiplier in {1/2, 1, 2, 4} x far-pile mode in {reference,
-split}.  Training data comes from the plain method itself: for each training
+actions: delta multiplier in {1/4, 1, 4, 16} x far-pile mode in {reference
+(one shared threshold), bucket (per-source thresholds)}.  Training data comes from the plain method itself: for each training
 instance the kernel is run under a sweep of forced per-phase schedules, the
 fastest one is kept, and the features it recorded at each phase are labelled
 with the action it took there (``sweep_instance``).  ``train`` fits the MLP with
@@ -25,8 +25,10 @@ from ._lib import check, lib
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 N_FEAT, N_HID, N_ACT, MAX_FORCED = 16, 64, 8, 256
-DELTA_MULTS = [0.5, 1.0, 2.0, 4.0]
-ACTIONS = [(m, far) for far in (0, 1) for m in DELTA_MULTS]  # action id = far * 4 + mult index
+DELTA_MULTS = [0.25, 1.0, 4.0, 16.0]
+FAR_REFERENCE, FAR_SPLIT, FAR_BUCKET = 0, 1, 2
+# action id = 4 * (far mode is bucket) + delta multiplier index
+ACTIONS = [(m, far) for far in (FAR_REFERENCE, FAR_BUCKET) for m in DELTA_MULTS]
 FIXED_ACTION = 1  # delta x1, reference far pile: the plain method's schedule
 
 OFF_MEAN = 0

@@ -62,10 +62,14 @@ enum {
                          takes one edge (the reference's fine-grained np) */
 };
 
-/* Far-pile modes. */
+/* Far-pile modes.  Every mode gives the same distances; they differ in which
+ * vertices a phase expands. */
 enum {
   LSG_FAR_REFERENCE = 0, /* whole far pile becomes the next phase (apps/sssp/kernel.cu:374-383) */
-  LSG_FAR_SPLIT = 1      /* far items above the next threshold wait for a later phase */
+  LSG_FAR_SPLIT = 1,     /* far items above the next threshold wait for a later phase */
+  LSG_FAR_BUCKET = 2     /* batched only: each source's threshold is its own smallest pending
+                            distance + delta - 1 (delta-stepping's lowest non-empty bucket), items
+                            above it wait */
 };
 
 /* Options of the sssp/bfs apps: -d, -s and -l (apps/sssp/support.cu:6-31,

@@ -35,7 +35,7 @@ def test_multi_small_golden(small_graphs):
         assert np.array_equal(ref[:, 0], gg["dist"])
         dev = g.copy_to_gpu()
         for delta in (1, 5, 32):
-            for far in (0, 1):
+            for far in (0, 1, 2):
                 say("multi small", name, delta, far)
                 d, _ = sssp_multi(dev, srcs, Options(delta=delta, far_mode=far))
                 assert np.array_equal(d, ref), (name, delta, far)
@@ -51,8 +51,9 @@ def rmat16():
     return g, srcs, oracle_cols(g, srcs)
 
 
-@pytest.mark.parametrize("opts", [dict(), dict(far_mode=1), dict(block_size=256, heavy_deg=256),
-                                  dict(block_size=1024, delta=7), dict(delta=10000)])
+@pytest.mark.parametrize("opts", [dict(), dict(far_mode=1), dict(far_mode=2), dict(far_mode=2, delta=1),
+                                  dict(block_size=256, heavy_deg=256), dict(block_size=1024, delta=7),
+                                  dict(delta=10000)])
 def test_multi_rmat16(rmat16, opts):
     g, srcs, ref = rmat16
     dev = g.copy_to_gpu()
