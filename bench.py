@@ -67,13 +67,22 @@ def main() -> int:
     row = g.row_start.astype(np.int64)
     deg = np.diff(row)
 
+    from galoisgpu_amd import autotune
+    tname = f"rmat{args.scale}_{args.nsrc}src"
+    tuned_b = autotune.tuned_options(tname, "batched") or {}
+    tuned_p = autotune.tuned_options(tname, "plain") or {}
+    launch_b = {k: v for k, v in tuned_b.items() if k in ("heavy_deg", "block_size")}
+    launch_p = {k: v for k, v in tuned_p.items() if k in ("expand_mode", "block_size")}
     use_learned = False
     model_path = lsched.default_model_path()
     if args.sched == "learned" or (args.sched == "auto" and os.path.exists(model_path)):
         lsched.load_model(model_path, device=0)
         use_learned = True
-    bopts = Options(delta=args.delta, sched=1 if use_learned else 0)
-    popts = Options(delta=args.delta)
+    # fixed batched schedule: the autotuner's choice (or the plain delta if never tuned)
+    fixed_kw = dict(tuned_b) if tuned_b else dict(delta=args.delta)
+    bopts = Options(delta=args.delta, sched=1, **launch_b) if use_learned else Options(**fixed_kw)
+    popts = dict(delta=args.delta, **launch_p)
+    log(f"batched options: {bopts}; plain options: {popts}")
 
     # ---- correctness gate (outside the timed region) ------------------------
     D = DeviceArray(V * args.nsrc * 4, 0)
@@ -83,7 +92,7 @@ def main() -> int:
     er = np.zeros(args.nsrc, dtype=np.int64)
     vr = np.zeros(args.nsrc, dtype=np.int64)
     for k, s in enumerate(srcs):
-        sssp_into(dev, plain_dist.ptr, Options(delta=args.delta, start_node=int(s)))
+        sssp_into(dev, plain_dist.ptr, Options(start_node=int(s), **popts))
         d = plain_dist.to_numpy(np.int32, V)
         if not np.array_equal(d, Dh[:, k]):
             bad = int(np.count_nonzero(d != Dh[:, k]))
@@ -112,16 +121,29 @@ def main() -> int:
     achieved = bytes_per_step / (kernel_ms / 1000.0) / 1e9
     log(f"batched: {ms_step:.3f} ms/step wall, kernel {kernel_ms:.3f} ms, {gteps:.2f} GTEPS")
 
+    # ---- the autotuned fixed-schedule batched run, for comparison -----------
+    fixed = None
+    if use_learned:
+        fo = Options(**fixed_kw)
+        sssp_multi_into(dev, srcs, fo, D)
+        assert np.array_equal(D.to_numpy(np.int32, (V, args.nsrc)), Dh), "fixed schedule changed distances"
+        t3 = time.perf_counter()
+        fk = [sssp_multi_into(dev, srcs, fo, D)["ms_kernel"] for _ in range(args.steps)]
+        fwall = time.perf_counter() - t3
+        fixed = {"gteps": edges_per_step * args.steps / fwall / 1e9, "ms_per_step": fwall * 1000.0 / args.steps,
+                 "kernel_ms": float(np.mean(fk)), "options": fixed_kw}
+        log(f"batched fixed {fixed_kw}: {fixed['ms_per_step']:.3f} ms/step, {fixed['gteps']:.2f} GTEPS")
+
     # ---- plain kernel on the same GPU and sources ---------------------------
     plain = None
     if not args.no_plain:
         for s in srcs[: min(4, len(srcs))]:
-            sssp_into(dev, plain_dist.ptr, Options(delta=args.delta, start_node=int(s)))
+            sssp_into(dev, plain_dist.ptr, Options(start_node=int(s), **popts))
         pk = 0.0
         t2 = time.perf_counter()
         for _ in range(args.steps):
             for s in srcs:
-                st = sssp_into(dev, plain_dist.ptr, Options(delta=args.delta, start_node=int(s)))
+                st = sssp_into(dev, plain_dist.ptr, Options(start_node=int(s), **popts))
                 pk += st["ms_kernel"]
         pwall = time.perf_counter() - t2
         plain = {"gteps": edges_per_step * args.steps / pwall / 1e9, "ms_per_step": pwall * 1000.0 / args.steps,
@@ -164,8 +186,10 @@ def main() -> int:
         "config": {"workload": f"rmat{args.scale}_ef16_{args.nsrc}src_delta{args.delta}", "scale": args.scale,
                    "edge_factor": 16, "nsrc": args.nsrc, "delta": args.delta, "weights": "U[1,255]",
                    "seed": args.seed, "nnodes": V, "nedges": E, "sum_E_r": int(er.sum()),
-                   "scheduler": "learned" if use_learned else "fixed", "parallelism": "single GPU"},
+                   "scheduler": "learned" if use_learned else "fixed", "batched_options": str(bopts),
+                   "plain_options": popts, "parallelism": "single GPU"},
         "plain": plain,
+        "batched_fixed_schedule": fixed,
         "speedup_vs_plain": round(gteps / plain["gteps"], 3) if plain else None,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},

@@ -2,8 +2,9 @@
 
 The model (layout in csrc/sched_mlp.h) maps 16 frontier features, taken by the
 persistent kernel's barrier leader at every phase boundary, to one of 8
-actions: delta multiplier in {1/4, 1, 4, 16} x far-pile mode in {reference
-(one shared threshold), bucket (per-source thresholds)}.  Training data comes from the plain method itself: for each training
+actions: delta multiplier in {1/4, 1, 8, 2^24 (unbounded)} x far-pile mode in
+{reference (one shared threshold), bucket (per-source thresholds)}.  Training data comes
+from the plain method itself: for each training
 instance the kernel is run under a sweep of forced per-phase schedules, the
 fastest one is kept, and the features it recorded at each phase are labelled
 with the action it took there (``sweep_instance``).  ``train`` fits the MLP with
@@ -25,7 +26,7 @@ from ._lib import check, lib
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 N_FEAT, N_HID, N_ACT, MAX_FORCED = 16, 64, 8, 256
-DELTA_MULTS = [0.25, 1.0, 4.0, 16.0]
+DELTA_MULTS = [0.25, 1.0, 8.0, float(1 << 24)]  # the last one makes a phase plain Bellman-Ford rounds
 FAR_REFERENCE, FAR_SPLIT, FAR_BUCKET = 0, 1, 2
 # action id = 4 * (far mode is bucket) + delta multiplier index
 ACTIONS = [(m, far) for far in (FAR_REFERENCE, FAR_BUCKET) for m in DELTA_MULTS]
@@ -126,6 +127,128 @@ def train(X: np.ndarray, y: np.ndarray, w: Optional[np.ndarray] = None, steps: i
     return m, float(loss.value)
 
 
+def run_schedule(dev, srcs, base_delta: int, schedule: Sequence[int], model: np.ndarray, device: int = 0,
+                 repeats: int = 2, **opt) -> Tuple[float, np.ndarray]:
+    """Runs the batched kernel under a forced per-phase schedule.  Returns the
+    best kernel time (ms) over `repeats` runs and the per-phase feature records."""
+    from .apps import Options
+    from .batch import DeviceArray, last_features, sssp_multi_into
+    set_model(forced(model, schedule), device)
+    D = DeviceArray(dev.nnodes * len(srcs) * 4, device)
+    try:
+        best = float("inf")
+        feats = None
+        for _ in range(repeats):
+            st = sssp_multi_into(dev, srcs, Options(delta=base_delta, sched=1, trace=True, **opt), D)
+            if st["ms_kernel"] < best:
+                best = st["ms_kernel"]
+                feats = last_features(device)
+        return best, feats
+    finally:
+        D.close()
+
+
+def sweep_instance(dev, srcs, base_delta: int, model: np.ndarray, max_greedy_phases: int = 12, device: int = 0,
+                   log=None, **opt) -> Tuple[np.ndarray, np.ndarray, dict]:
+    """Labels one training instance by sweeping schedules of the plain method.
+
+    Every constant action is timed; then, phase by phase (up to
+    max_greedy_phases), each action is tried at that phase with the best
+    schedule found so far elsewhere, and the faster one is kept.  Returns the
+    fastest schedule's per-phase features X [P][16], its actions y [P] and a
+    summary (times of the fixed plain schedule, the best constant and the best
+    schedule)."""
+    times = {}
+
+    def timed(sch):
+        key = tuple(sch)
+        if key not in times:
+            times[key] = run_schedule(dev, srcs, base_delta, sch, model, device, **opt)
+        return times[key]
+
+    fixed_ms, _ = timed([FIXED_ACTION])
+    const = {a: timed([a])[0] for a in range(N_ACT)}
+    best_const = min(const, key=const.get)
+    best = [best_const]
+    best_ms, feats = timed(best)
+    nph = len(feats)
+    def extend(sch, n):
+        return [sch[min(i, len(sch) - 1)] for i in range(n)]
+
+    for p in range(min(nph, max_greedy_phases)):
+        for a in range(N_ACT):
+            cand = extend(best, max(len(best), p + 2))
+            cand[p] = a
+            ms, f = timed(cand)
+            if ms < best_ms * 0.995:
+                best, best_ms, feats = cand, ms, f
+        if log:
+            log(f"  phase {p}: best {best_ms:.2f} ms schedule {best[:p + 2]}")
+    X = feats[:, :N_FEAT].copy()
+    y = feats[:, N_FEAT].astype(np.int32)
+    summary = {"fixed_ms": fixed_ms, "best_const": int(best_const), "best_const_ms": const[best_const],
+               "best_ms": best_ms, "schedule": [int(a) for a in best], "phases": int(len(y)), "runs": len(times)}
+    return X, y, summary
+
+
+def training_instances(quick: bool = False) -> List[dict]:
+    """Graphs the scheduler is trained on (none of them is a benchmark graph)."""
+    inst = []
+    for scale, seed in ((14, 11), (16, 12), (17, 13), (18, 14)):
+        inst.append(dict(kind="rmat", scale=scale, seed=seed, nsrc=64, delta=32))
+    inst.append(dict(kind="rmat", scale=16, seed=15, nsrc=32, delta=256))
+    inst.append(dict(kind="grid", rows=256, cols=256, seed=16, nsrc=64, delta=32))
+    inst.append(dict(kind="road", rows=512, cols=512, seed=17, nsrc=64, delta=1000))
+    inst.append(dict(kind="road", rows=384, cols=384, seed=18, nsrc=32, delta=5000))
+    return inst[:3] if quick else inst
+
+
+def build_instance(spec: dict):
+    from .graph import CSRGraph
+    if spec["kind"] == "rmat":
+        g = CSRGraph.rmat(spec["scale"], 16, seed=spec["seed"])
+    elif spec["kind"] == "grid":
+        g = CSRGraph.grid(spec["rows"], spec["cols"], 1, 100, seed=spec["seed"])
+    else:
+        g = CSRGraph.roadlike(spec["rows"], spec["cols"], 0.10, 0.001, 1, 10000, seed=spec["seed"])
+    rng = np.random.default_rng(spec["seed"] + 7)
+    deg = g.out_degree()
+    cand = np.flatnonzero(deg > 0)
+    srcs = rng.choice(cand, size=spec["nsrc"], replace=False).astype(np.uint32)
+    return g, srcs
+
+
+def train_from_sweeps(out_path: str, quick: bool = False, device: int = 0, steps: int = 6000, log=print) -> dict:
+    """Full pipeline: sweep every training instance, train the MLP with the
+    HIP trainer, save the model with its provenance."""
+    base = new_model(0)
+    Xs, ys, ws, summaries = [], [], [], []
+    for spec in training_instances(quick):
+        g, srcs = build_instance(spec)
+        dev = g.copy_to_gpu(device)
+        t0 = time.time()
+        X, y, summ = sweep_instance(dev, srcs, spec["delta"], base, device=device, log=log)
+        dev.close()
+        gain = summ["fixed_ms"] / max(summ["best_ms"], 1e-6)
+        summ.update(spec=spec, sweep_s=round(time.time() - t0, 1), gain=round(gain, 3))
+        log(f"instance {spec}: fixed {summ['fixed_ms']:.2f} ms, best const a{summ['best_const']} "
+            f"{summ['best_const_ms']:.2f} ms, best {summ['best_ms']:.2f} ms ({summ['runs']} runs)")
+        Xs.append(X)
+        ys.append(y)
+        ws.append(np.full(len(y), 1.0 + np.log(gain), dtype=np.float32))
+        summaries.append(summ)
+    X = np.concatenate(Xs)
+    y = np.concatenate(ys)
+    w = np.concatenate(ws)
+    model, loss = train(X, y, w, steps=steps, device=device)
+    acc = float((infer(model, X, device) == y).mean())
+    meta = {"instances": summaries, "samples": int(len(y)), "train_loss": loss, "train_acc": acc,
+            "actions": ACTIONS, "steps": steps, "created": time.strftime("%Y-%m-%d %H:%M:%S")}
+    save_model(out_path, model, meta)
+    log(f"trained on {len(y)} phase samples: loss {loss:.4f}, accuracy {acc:.3f} -> {out_path}")
+    return meta
+
+
 def profiled_traffic(root: str, name: str) -> Optional[float]:
     """HBM bytes per launch of kernel/workload `name` from the committed PMC
     summary profiles/<name>_pmc.txt (line 'traffic_bytes_per_launch: N'), or None."""
@@ -143,3 +266,15 @@ def profiled_traffic(root: str, name: str) -> Optional[float]:
 
 __all__ = ["ACTIONS", "FIXED_ACTION", "MODEL_FLOATS", "new_model", "forced", "set_model", "save_model",
            "load_model", "infer", "train", "default_model_path", "profiled_traffic"]
+
+
+if __name__ == "__main__":
+    import argparse
+
+    ap = argparse.ArgumentParser(description="train the learned phase scheduler on the GPU")
+    ap.add_argument("command", choices=["train"])
+    ap.add_argument("--out", default=default_model_path())
+    ap.add_argument("--quick", action="store_true")
+    ap.add_argument("--steps", type=int, default=6000)
+    a = ap.parse_args()
+    train_from_sweeps(a.out, quick=a.quick, steps=a.steps, log=lambda *x: print(*x, flush=True))
