@@ -68,7 +68,9 @@ class LsgOpts(ctypes.Structure):
         ("heavy_deg", c_int32),
         ("sched", c_int32),
         ("trace", c_int32),
-        ("reserved", c_int32 * 5),
+        ("graph_launch", c_int32),
+        ("pull_permille", c_int32),
+        ("reserved", c_int32 * 3),
     ]
 
 
@@ -116,6 +118,7 @@ class LsgBatchStats(ctypes.Structure):
         ("heavy_chunks", c_uint64),
         ("lane_items", c_uint64),
         ("edges_expanded", c_uint64),
+        ("improved_rows", c_uint64),
         ("grid_blocks", c_int32),
         ("block_size", c_int32),
         ("launches", c_int32),
@@ -162,6 +165,8 @@ SIGNATURES = {
                                POINTER(LsgBatchStats)]),
     "lsg_sssp_batch": (c_int, [POINTER(LsgGraph), c_void_p, c_uint32, POINTER(LsgOpts), c_void_p,
                                POINTER(LsgBatchStats)]),
+    "lsg_sssp_graphs": (c_int, [POINTER(LsgGraph), c_uint32, c_void_p, POINTER(LsgOpts), c_void_p,
+                                POINTER(LsgBatchStats)]),
     "lsg_last_features": (c_int, [c_int, c_void_p, c_uint32, POINTER(c_uint32)]),
     "lsg_sched_model_floats": (c_int, []),
     "lsg_sched_set_model": (c_int, [c_int, c_void_p, c_uint32]),
@@ -171,6 +176,7 @@ SIGNATURES = {
     "lsg_dev_malloc": (c_int, [c_int, c_size_t, POINTER(c_void_p)]),
     "lsg_dev_free": (c_int, [c_int, c_void_p]),
     "lsg_dev_copy": (c_int, [c_int, c_void_p, c_void_p, c_size_t, c_int]),
+    "lsg_stream_copy": (c_int, [c_int, c_size_t, c_int, POINTER(c_double)]),
     "lsg_set_device": (c_int, [c_int]),
     "lsg_device_name": (c_int, [c_int, c_char_p, c_size_t]),
     "lsg_device_sync": (c_int, [c_int]),

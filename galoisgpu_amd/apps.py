@@ -32,6 +32,8 @@ class Options:
     heavy_deg: int = 0
     sched: int = 0
     trace: bool = False
+    graph_launch: bool = False
+    pull_permille: int = 0
 
     def to_c(self) -> LsgOpts:
         o = LsgOpts()

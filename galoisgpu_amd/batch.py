@@ -14,7 +14,7 @@ from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 
-from ._lib import LsgBatchStats, check, lib
+from ._lib import LsgBatchStats, LsgGraph, check, lib
 from .apps import Options
 from .graph import DeviceGraph
 
@@ -97,6 +97,27 @@ def sssp_batch(dev: DeviceGraph, srcs: Sequence[int], opts: Optional[Options] = 
             o.close()
 
 
+def sssp_graphs(devs: Sequence[DeviceGraph], srcs: Sequence[int], opts: Optional[Options] = None
+                ) -> Tuple[List[np.ndarray], dict]:
+    """Many graphs, one source each, in one persistent launch (disjoint-union multi-seed run)."""
+    opts = opts or Options()
+    n = len(devs)
+    if len(srcs) != n:
+        raise ValueError("one source per graph")
+    arr = (LsgGraph * n)(*[d._g for d in devs])
+    s = np.ascontiguousarray(np.asarray(srcs, dtype=np.uint32))
+    outs = [DeviceArray(d.nnodes * 4, d.device) for d in devs]
+    try:
+        ptrs = (ctypes.c_void_p * n)(*[o.ptr for o in outs])
+        st = LsgBatchStats()
+        check(lib().lsg_sssp_graphs(arr, n, s.ctypes.data_as(ctypes.c_void_p), ctypes.byref(opts.to_c()), ptrs,
+                                    ctypes.byref(st)), "sssp_graphs")
+        return [o.to_numpy(np.int32, d.nnodes) for o, d in zip(outs, devs)], st.as_dict()
+    finally:
+        for o in outs:
+            o.close()
+
+
 def last_features(device: int = 0) -> np.ndarray:
     """Per-phase records of the last traced batched call: [nphases][20] float32."""
     n = ctypes.c_uint32()
@@ -108,4 +129,4 @@ def last_features(device: int = 0) -> np.ndarray:
     return out
 
 
-__all__ = ["DeviceArray", "sssp_multi", "sssp_multi_into", "sssp_batch", "last_features"]
+__all__ = ["DeviceArray", "sssp_multi", "sssp_multi_into", "sssp_batch", "sssp_graphs", "last_features"]

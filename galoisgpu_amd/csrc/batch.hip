@@ -461,7 +461,7 @@ static MultiFn pick_multi(int bs) {
 
 struct MultiRun {
   double ms_kernel = 0;
-  uint64_t rounds = 0, phases = 0, items = 0, heavy = 0, lane_items = 0, edges = 0;
+  uint64_t rounds = 0, phases = 0, items = 0, heavy = 0, lane_items = 0, edges = 0, improved = 0;
   int grid = 0, bs = 0;
 };
 
@@ -531,12 +531,25 @@ static int run_multi(Workspace *w, const lsg_graph *g, const uint32_t *srcs_host
   Ctrl hc;
   float ms = 0;
   for (int attempt = 0;; ++attempt) {
-    HIPCHK(hipEventRecord(w->ev0, w->stream));
     uint32_t init_blocks = (uint32_t)std::min<uint64_t>(((uint64_t)V * nsrc + 255) / 256, 16ull * w->cus);
-    hipLaunchKernelGGL(multi_init, dim3(init_blocks), dim3(256), 0, w->stream, a, tot);
-    hipLaunchKernelGGL(multi_seed, dim3(1), dim3(64), 0, w->stream, a);
-    hipLaunchKernelGGL(fn, dim3(grid), dim3(bs), 0, w->stream, a, tot);
-    HIPCHK(hipGetLastError());
+    auto enqueue = [&]() -> int {
+      hipLaunchKernelGGL(multi_init, dim3(init_blocks), dim3(256), 0, w->stream, a, tot);
+      hipLaunchKernelGGL(multi_seed, dim3(1), dim3(64), 0, w->stream, a);
+      hipLaunchKernelGGL(fn, dim3(grid), dim3(bs), 0, w->stream, a, tot);
+      HIPCHK(hipGetLastError());
+      return LSG_OK;
+    };
+    HIPCHK(hipEventRecord(w->ev0, w->stream));
+    if (o.graph_launch) {
+      std::vector<uint8_t> key;
+      key_append(key, a);
+      key_append(key, tot);
+      key_append(key, grid);
+      key_append(key, bs);
+      if ((rc = launch_captured(w, key, enqueue))) return rc;
+    } else if ((rc = enqueue())) {
+      return rc;
+    }
     HIPCHK(hipEventRecord(w->ev1, w->stream));
     HIPCHK(hipMemcpyAsync(&hc, w->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, w->stream));
     HIPCHK(hipStreamSynchronize(w->stream));
@@ -570,6 +583,7 @@ static int run_multi(Workspace *w, const lsg_graph *g, const uint32_t *srcs_host
   out->heavy = hc.stat_heavy;
   out->lane_items = ht.lane_items;
   out->edges = ht.edges;
+  out->improved = ht.improved;
   out->grid = grid;
   out->bs = bs;
   return LSG_OK;
@@ -592,6 +606,7 @@ static void fill_stats(lsg_batch_stats *st, const MultiRun &r, int launches) {
   st->heavy_chunks += r.heavy;
   st->lane_items += r.lane_items;
   st->edges_expanded += r.edges;
+  st->improved_rows += r.improved;
   st->grid_blocks = r.grid;
   st->block_size = r.bs;
   st->launches = launches;

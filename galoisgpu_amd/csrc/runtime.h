@@ -19,7 +19,18 @@
 
 namespace lsg {
 
-enum BufSlot { kBufQueues = 0, kBufHeavy, kBufStamps, kBufMasks, kBufTrace, kBufTranspose, kBufSeeds, kBufCount };
+enum BufSlot {
+  kBufQueues = 0,
+  kBufHeavy,
+  kBufStamps,
+  kBufMasks,
+  kBufTrace,
+  kBufTranspose,
+  kBufSeeds,
+  kBufSeedList,
+  kBufUnion,
+  kBufCount
+};
 
 struct Workspace {
   int device = -1;
@@ -37,8 +48,51 @@ struct Workspace {
   std::vector<GridEntry> grids;             // calibrated persistent grid per kernel
   std::vector<lsg_round_trace> last_trace;  // host copy of the last traced call
   std::vector<float> last_feat;             // per-phase feature records of the last traced batched call
+  struct Captured {
+    std::vector<uint8_t> key;  // exact bytes of the launch arguments
+    hipGraphExec_t exec;
+  };
+  std::vector<Captured> captured;           // hipGraph launch sequences, most recent last
   std::mutex mu;                            // one traversal at a time per device
 };
+
+// Launches the kernel sequence that `enqueue` puts on w->stream as one
+// hipGraph: captured and instantiated the first time a key is seen, replayed
+// afterwards.  `key` must hold every byte the kernels' arguments depend on.
+template <class F>
+int launch_captured(Workspace *w, const std::vector<uint8_t> &key, F enqueue) {
+  for (auto &c : w->captured)
+    if (c.key == key) {
+      HIPCHK(hipGraphLaunch(c.exec, w->stream));
+      return LSG_OK;
+    }
+  hipGraph_t graph = nullptr;
+  HIPCHK(hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal));
+  int rc = enqueue();
+  hipError_t e = hipStreamEndCapture(w->stream, &graph);
+  if (rc) {
+    if (e == hipSuccess && graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(LSG_ERR_HIP, "stream capture failed: %s", hipGetErrorString(e));
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return fail(LSG_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+  if (w->captured.size() >= 16) {
+    (void)hipGraphExecDestroy(w->captured.front().exec);
+    w->captured.erase(w->captured.begin());
+  }
+  w->captured.push_back({key, exec});
+  HIPCHK(hipGraphLaunch(exec, w->stream));
+  return LSG_OK;
+}
+
+template <class T>
+void key_append(std::vector<uint8_t> &k, const T &v) {
+  const uint8_t *p = reinterpret_cast<const uint8_t *>(&v);
+  k.insert(k.end(), p, p + sizeof(T));
+}
 
 // Returns the workspace of `device`, creating it on first use.
 int get_workspace(int device, Workspace **out);

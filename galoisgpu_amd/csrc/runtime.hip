@@ -83,9 +83,50 @@ void record_resident_grid(Workspace *w, const void *fn, int bs, int grid) {
 
 }  // namespace lsg
 
+namespace lsg {
+
+// Plain 16-byte-per-lane copy, grid-stride: the HBM reference figure and the
+// known-size kernel that PMC byte counters are calibrated against.
+__global__ void stream_copy_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n) {
+  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x, st = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += st) dst[i] = src[i];
+}
+
+}  // namespace lsg
+
 using namespace lsg;
 
 extern "C" {
+
+int lsg_stream_copy(int device, size_t bytes, int iters, double *ms_per_copy) {
+  if (bytes < 16 || iters <= 0 || !ms_per_copy) return fail(LSG_ERR_ARG, "stream_copy: bad arguments");
+  Workspace *w;
+  int rc = get_workspace(device, &w);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(w->mu);
+  HIPCHK(hipSetDevice(device));
+  size_t n = bytes / 16;
+  void *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, n * 16) != hipSuccess || hipMalloc(&b, n * 16) != hipSuccess) {
+    if (a) (void)hipFree(a);
+    return fail(LSG_ERR_NOMEM, "stream_copy: %zu bytes unavailable", 2 * n * 16);
+  }
+  HIPCHK(hipMemsetAsync(a, 1, n * 16, w->stream));
+  uint32_t blocks = (uint32_t)std::min<size_t>((n + 255) / 256, (size_t)w->cus * 8);
+  hipLaunchKernelGGL(stream_copy_kernel, dim3(blocks), dim3(256), 0, w->stream, (const uint4 *)a, (uint4 *)b, n);
+  HIPCHK(hipEventRecord(w->ev0, w->stream));
+  for (int i = 0; i < iters; ++i)
+    hipLaunchKernelGGL(stream_copy_kernel, dim3(blocks), dim3(256), 0, w->stream, (const uint4 *)a, (uint4 *)b, n);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(w->ev1, w->stream));
+  HIPCHK(hipEventSynchronize(w->ev1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, w->ev0, w->ev1));
+  *ms_per_copy = ms / iters;
+  HIPCHK(hipFree(a));
+  HIPCHK(hipFree(b));
+  return LSG_OK;
+}
 
 int lsg_dev_malloc(int device, size_t bytes, void **ptr) {
   if (!ptr) return fail(LSG_ERR_ARG, "dev_malloc: NULL out pointer");

@@ -179,6 +179,17 @@ __device__ void process_tile(const KArgs &a, const Sched &s, uint32_t t, TileSha
     du = ld_coh(a.dist + u);
     if (du == kInf) valid = false;  // INF nodes are never expanded (apps/sssp/kernel.cu:202)
   }
+  if (A == Algo::SSSP && s.aux[0] && s.aux[1] == LSG_FAR_SPLIT) {
+    // Split far pile: an item still above this phase's threshold re-enters
+    // the far pile instead of expanding early.
+    bool defer = valid && (int64_t)du > s.curdelta();
+    if (defer) {
+      uint32_t stamp = s.phase + 1;
+      defer = a.far_stamp[u] != stamp && atomic_exch(a.far_stamp + u, stamp) != stamp;
+      valid = false;
+    }
+    wave_push(defer, u, a.q[s.far_q], &a.ctrl->qcount[s.far_q], a.qcap, &a.ctrl->abort);
+  }
   if (valid) {
     rs = a.row_start[u];
     deg = a.row_start[u + 1] - rs;
@@ -314,12 +325,14 @@ __device__ void leader_step(const KArgs &a) {
     s.in_count = nnear;
     s.near_q = consumed;
     st_coh(&c->qcount[consumed], 0u);
+    s.aux[0] = 0;
     s.round += 1;
   } else if (A == Algo::SSSP && nfar > 0) {
     s.in_q = s.far_q;
     s.in_count = nfar;
     s.far_q = consumed;
     st_coh(&c->qcount[consumed], 0u);
+    s.aux[0] = 1;  // this round pops the far pile
     s.set_curdelta(s.curdelta() + s.delta);
     s.phase += 1;
     s.round += 1;
@@ -368,12 +381,12 @@ __global__ __launch_bounds__(BS) void persistent_kernel(KArgs a) {
   }
 }
 
-// Initialise distances, stamps and the control block; seed the in-queue with
-// the source (apps/sssp/kernel.cu:23-35,469,476-477; apps/bfs/kernel.cu:20-32).
-__global__ void init_kernel(KArgs a, uint32_t src, uint32_t delta) {
+// Initialise distances, stamps and the control block (apps/sssp/kernel.cu:23-35,469;
+// apps/bfs/kernel.cu:20-32).  The sources are seeded by seed_kernel.
+__global__ void init_kernel(KArgs a, uint32_t delta) {
   uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
   for (uint32_t v = tid; v < a.nnodes; v += nt) {
-    a.dist[v] = v == src ? 0u : kInf;
+    a.dist[v] = kInf;
     a.near_stamp[v] = 0u;
     a.far_stamp[v] = 0u;
   }
@@ -381,7 +394,7 @@ __global__ void init_kernel(KArgs a, uint32_t src, uint32_t delta) {
     Ctrl *c = a.ctrl;
     c->bar_count = 0;
     c->bar_gen = 0;
-    c->qcount[0] = 1;
+    c->qcount[0] = 0;
     c->qcount[1] = 0;
     c->qcount[2] = 0;
     c->hcount[0] = 0;
@@ -395,13 +408,30 @@ __global__ void init_kernel(KArgs a, uint32_t src, uint32_t delta) {
     s.in_q = 0;
     s.near_q = 1;
     s.far_q = 2;
-    s.in_count = 1;
     s.round = 1;
     s.set_curdelta(0);  // apps/sssp/kernel.cu:472
     s.delta = delta;
+    s.aux[1] = a.far_mode;
     c->s = s;
     c->stat_items = c->stat_rounds = c->stat_heavy = c->stat_phases = 0;
-    a.q[0][0] = src;
+  }
+}
+
+// d[seed] = 0 and the seeds form the first in-queue (apps/sssp/kernel.cu:33,476-477).
+// One block.  Several seeds make a multi-seed traversal: on a disjoint union of
+// graphs with one seed each it computes every graph's own distances.
+__global__ void seed_kernel(KArgs a, const uint32_t *seeds, uint32_t nseeds) {
+  __shared__ uint32_t cnt;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < nseeds; i += blockDim.x) {
+    uint32_t v = seeds[i];
+    if (v < a.nnodes && atomicExch(a.dist + v, 0u) != 0u) a.q[0][atomicAdd(&cnt, 1u)] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    a.ctrl->qcount[0] = cnt;
+    a.ctrl->s.in_count = cnt;
   }
 }
 
@@ -418,53 +448,59 @@ static KernelFn pick_kernel(int bs) {
   }
 }
 
-// Runs one traversal into dist (device array of g->nnodes).  Synchronous.
+// The arrays of one traversal: a device CSR, possibly a packed union of graphs.
+struct CsrView {
+  const uint32_t *row, *dst, *w;
+  uint32_t nnodes;
+  uint64_t nedges;
+};
+
+static int check_single_opts(const lsg_opts &o, bool sssp, int *bs, int *expand) {
+  if (sssp && o.delta <= 0) return fail(LSG_ERR_ARG, "delta must be > 0 (got %d)", o.delta);
+  *bs = o.block_size ? o.block_size : 256;
+  if (*bs != 256 && *bs != 512 && *bs != 1024) return fail(LSG_ERR_ARG, "block_size must be 256, 512 or 1024");
+  *expand = o.enable_lb ? LSG_EXPAND_LB : o.expand_mode;
+  if (*expand != LSG_EXPAND_TWB && *expand != LSG_EXPAND_LB) return fail(LSG_ERR_ARG, "bad expand_mode %d", *expand);
+  if (o.far_mode != LSG_FAR_REFERENCE && o.far_mode != LSG_FAR_SPLIT)
+    return fail(LSG_ERR_ARG, "far_mode %d is not available for single-source runs (reference or split)", o.far_mode);
+  return LSG_OK;
+}
+
+// Runs one traversal from `nseeds` seeds (host array) into dist (device,
+// g.nnodes entries).  Synchronous.  With graph_launch the init, seed and
+// persistent kernels are replayed as one captured hipGraph.
 template <Algo A>
-static int run_single(const lsg_graph *g, const lsg_opts *o_in, uint32_t *dist, lsg_stats *st) {
+static int run_core(Workspace *w, const CsrView &g, const uint32_t *seeds, uint32_t nseeds, const lsg_opts &o,
+                    uint32_t *dist, bool graph_launch, lsg_stats *st) {
   auto t0 = std::chrono::steady_clock::now();
-  if (!g || !g->device_graph) return fail(LSG_ERR_ARG, "expected a device graph (copy_to_gpu first)");
-  if (!dist) return fail(LSG_ERR_ARG, "NULL distance array");
-  lsg_opts o;
-  if (o_in) o = *o_in;
-  else lsg_opts_default(&o);
-  if (A == Algo::SSSP && o.delta <= 0) return fail(LSG_ERR_ARG, "delta must be > 0 (got %d)", o.delta);
-  if (o.start_node < 0 || (uint32_t)o.start_node >= g->nnodes)
-    return fail(LSG_ERR_ARG, "start node %d out of range [0, %u)", o.start_node, g->nnodes);
-  int bs = o.block_size ? o.block_size : 256;
-  if (bs != 256 && bs != 512 && bs != 1024) return fail(LSG_ERR_ARG, "block_size must be 256, 512 or 1024");
+  int bs, expand, rc;
+  if ((rc = check_single_opts(o, A == Algo::SSSP, &bs, &expand))) return rc;
   uint32_t thread_deg = o.thread_deg > 0 ? (uint32_t)o.thread_deg : 8u;
   uint32_t heavy_deg = o.heavy_deg > 0 ? (uint32_t)o.heavy_deg : 4096u;
   if (heavy_deg < (uint32_t)bs) heavy_deg = (uint32_t)bs;
-  int expand = o.enable_lb ? LSG_EXPAND_LB : o.expand_mode;
-  if (expand != LSG_EXPAND_TWB && expand != LSG_EXPAND_LB) return fail(LSG_ERR_ARG, "bad expand_mode %d", expand);
-
-  Workspace *w;
-  int rc = get_workspace(g->device, &w);
-  if (rc) return rc;
-  std::lock_guard<std::mutex> lk(w->mu);
-  HIPCHK(hipSetDevice(w->device));
-  uint64_t hcap64 = 2ull * g->nedges / heavy_deg + 64;
+  uint64_t hcap64 = 2ull * g.nedges / heavy_deg + 64;
   uint32_t hcap = (uint32_t)std::min<uint64_t>(hcap64, UINT32_MAX / 2);
-  void *qbuf, *hbuf, *sbuf;
-  if ((rc = ws_buffer(w, kBufQueues, 3ull * g->nnodes * 4, &qbuf)) ||
+  void *qbuf, *hbuf, *sbuf, *seedbuf;
+  if ((rc = ws_buffer(w, kBufQueues, 3ull * g.nnodes * 4, &qbuf)) ||
       (rc = ws_buffer(w, kBufHeavy, 2ull * hcap * 8, &hbuf)) ||
-      (rc = ws_buffer(w, kBufStamps, 2ull * g->nnodes * 4, &sbuf)))
+      (rc = ws_buffer(w, kBufStamps, 2ull * g.nnodes * 4, &sbuf)) ||
+      (rc = ws_buffer(w, kBufSeedList, (size_t)nseeds * 4, &seedbuf)))
     return rc;
 
   KArgs a;
   memset(&a, 0, sizeof a);
-  a.row_start = g->row_start;
-  a.edge_dst = g->edge_dst;
-  a.edge_data = (const uint32_t *)g->edge_data;
+  a.row_start = g.row;
+  a.edge_dst = g.dst;
+  a.edge_data = g.w;
   a.dist = dist;
-  for (int i = 0; i < 3; ++i) a.q[i] = (uint32_t *)qbuf + (size_t)i * g->nnodes;
+  for (int i = 0; i < 3; ++i) a.q[i] = (uint32_t *)qbuf + (size_t)i * g.nnodes;
   a.hq[0] = (uint32_t *)hbuf;
   a.hq[1] = (uint32_t *)hbuf + 2ull * hcap;
   a.near_stamp = (uint32_t *)sbuf;
-  a.far_stamp = (uint32_t *)sbuf + g->nnodes;
+  a.far_stamp = (uint32_t *)sbuf + g.nnodes;
   a.ctrl = w->ctrl;
-  a.nnodes = g->nnodes;
-  a.qcap = g->nnodes;
+  a.nnodes = g.nnodes;
+  a.qcap = g.nnodes;
   a.hcap = hcap;
   a.delta = A == Algo::SSSP ? (uint32_t)o.delta : 1u;
   a.thread_deg = thread_deg;
@@ -472,7 +508,6 @@ static int run_single(const lsg_graph *g, const lsg_opts *o_in, uint32_t *dist, 
   a.expand_mode = (uint32_t)expand;
   a.far_mode = (uint32_t)o.far_mode;
   a.spin_limit = spin_limit_ticks(w);
-
   a.residency_limit = residency_limit_ticks(w);
   if (o.trace) {
     void *tbuf;
@@ -480,22 +515,37 @@ static int run_single(const lsg_graph *g, const lsg_opts *o_in, uint32_t *dist, 
     a.trace = (lsg_round_trace *)tbuf;
     a.trace_cap = kTraceCap;
   }
+  HIPCHK(hipMemcpyAsync(seedbuf, seeds, (size_t)nseeds * 4, hipMemcpyHostToDevice, w->stream));
 
   KernelFn fn = A == Algo::SSSP ? pick_kernel<Algo::SSSP>(bs) : pick_kernel<Algo::BFS>(bs);
   int grid = 0;
-  rc = resident_grid(w, (const void *)fn, bs, &grid);
-  if (rc) return rc;
+  if ((rc = resident_grid(w, (const void *)fn, bs, &grid))) return rc;
   if (o.grid_blocks > 0) grid = std::min(o.grid_blocks, grid);
 
   Ctrl hc;
   float ms = 0;
   for (int attempt = 0;; ++attempt) {
+    const uint32_t init_blocks = std::min<uint32_t>((g.nnodes + 255) / 256, 8u * w->cus);
+    const uint32_t *sd = (const uint32_t *)seedbuf;
+    auto enqueue = [&]() -> int {
+      hipLaunchKernelGGL(init_kernel, dim3(init_blocks), dim3(256), 0, w->stream, a, a.delta);
+      hipLaunchKernelGGL(seed_kernel, dim3(1), dim3(256), 0, w->stream, a, sd, nseeds);
+      hipLaunchKernelGGL(fn, dim3(grid), dim3(bs), 0, w->stream, a);
+      HIPCHK(hipGetLastError());
+      return LSG_OK;
+    };
     HIPCHK(hipEventRecord(w->ev0, w->stream));
-    uint32_t init_blocks = std::min<uint32_t>((g->nnodes + 255) / 256, 8u * w->cus);
-    hipLaunchKernelGGL(init_kernel, dim3(init_blocks), dim3(256), 0, w->stream, a, (uint32_t)o.start_node, a.delta);
-    HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(fn, dim3(grid), dim3(bs), 0, w->stream, a);
-    HIPCHK(hipGetLastError());
+    if (graph_launch) {
+      std::vector<uint8_t> key;
+      key_append(key, (int)A);
+      key_append(key, a);
+      key_append(key, grid);
+      key_append(key, bs);
+      key_append(key, nseeds);
+      if ((rc = launch_captured(w, key, enqueue))) return rc;
+    } else if ((rc = enqueue())) {
+      return rc;
+    }
     HIPCHK(hipEventRecord(w->ev1, w->stream));
     HIPCHK(hipMemcpyAsync(&hc, w->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, w->stream));
     HIPCHK(hipStreamSynchronize(w->stream));
@@ -533,6 +583,84 @@ static int run_single(const lsg_graph *g, const lsg_opts *o_in, uint32_t *dist, 
     st->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }
   return LSG_OK;
+}
+
+// One source on one device graph into dist (device array of g->nnodes).
+template <Algo A>
+static int run_single(const lsg_graph *g, const lsg_opts *o_in, uint32_t *dist, lsg_stats *st) {
+  auto t0 = std::chrono::steady_clock::now();
+  if (!g || !g->device_graph) return fail(LSG_ERR_ARG, "expected a device graph (copy_to_gpu first)");
+  if (!dist) return fail(LSG_ERR_ARG, "NULL distance array");
+  lsg_opts o;
+  if (o_in) o = *o_in;
+  else lsg_opts_default(&o);
+  if (o.start_node < 0 || (uint32_t)o.start_node >= g->nnodes)
+    return fail(LSG_ERR_ARG, "start node %d out of range [0, %u)", o.start_node, g->nnodes);
+  Workspace *w;
+  int rc = get_workspace(g->device, &w);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(w->mu);
+  HIPCHK(hipSetDevice(w->device));
+  CsrView v{g->row_start, g->edge_dst, (const uint32_t *)g->edge_data, g->nnodes, g->nedges};
+  uint32_t seed = (uint32_t)o.start_node;
+  rc = run_core<A>(w, v, &seed, 1, o, dist, o.graph_launch != 0, st);
+  if (rc == LSG_OK && st)
+    st->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return rc;
+}
+
+// ---- many graphs in one launch -----------------------------------------------
+
+// One member graph of a packed union: its arrays, output, and offsets.
+struct UnionMember {
+  const uint32_t *row, *dst, *w;
+  int32_t *out;
+  uint32_t nnodes, voff;
+  uint64_t nedges, eoff;
+};
+
+__device__ __forceinline__ uint32_t member_of_vertex(const UnionMember *m, uint32_t n, uint32_t v) {
+  uint32_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (m[mid].voff <= v) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ uint32_t member_of_edge(const UnionMember *m, uint32_t n, uint64_t e) {
+  uint32_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (m[mid].eoff <= e) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Disjoint union: vertex ids shifted by voff, edge offsets by eoff.
+__global__ void pack_union_kernel(const UnionMember *m, uint32_t n, uint32_t *urow, uint32_t *udst, uint32_t *uw,
+                                  uint32_t V, uint64_t E) {
+  size_t tid = blockIdx.x * (size_t)blockDim.x + threadIdx.x, nt = (size_t)gridDim.x * blockDim.x;
+  for (size_t v = tid; v < V; v += nt) {
+    const UnionMember &g = m[member_of_vertex(m, n, (uint32_t)v)];
+    urow[v] = (uint32_t)(g.row[v - g.voff] + g.eoff);
+  }
+  if (tid == 0) urow[V] = (uint32_t)E;
+  for (size_t e = tid; e < E; e += nt) {
+    const UnionMember &g = m[member_of_edge(m, n, e)];
+    udst[e] = g.dst[e - g.eoff] + g.voff;
+    uw[e] = g.w[e - g.eoff];
+  }
+}
+
+__global__ void unpack_union_kernel(const UnionMember *m, uint32_t n, const uint32_t *udist, uint32_t V) {
+  size_t tid = blockIdx.x * (size_t)blockDim.x + threadIdx.x, nt = (size_t)gridDim.x * blockDim.x;
+  for (size_t v = tid; v < V; v += nt) {
+    const UnionMember &g = m[member_of_vertex(m, n, (uint32_t)v)];
+    g.out[v - g.voff] = (int32_t)udist[v];
+  }
 }
 
 }  // namespace lsg
@@ -642,6 +770,67 @@ int lsg_sssp_dist(const lsg_graph *gg, const lsg_opts *opts, int32_t *dist_dev, 
 int lsg_bfs_dist(const lsg_graph *gg, const lsg_opts *opts, int32_t *level_dev, lsg_stats *stats) {
   clear_error();
   return run_single<Algo::BFS>(gg, opts, (uint32_t *)level_dev, stats);
+}
+
+int lsg_sssp_graphs(const lsg_graph *graphs, uint32_t ngraphs, const uint32_t *srcs, const lsg_opts *opts,
+                    int32_t *const *dist_out, lsg_batch_stats *stats) {
+  clear_error();
+  auto t0 = std::chrono::steady_clock::now();
+  if (!graphs || !srcs || !dist_out || ngraphs == 0) return fail(LSG_ERR_ARG, "sssp_graphs: empty or NULL input");
+  lsg_opts o;
+  if (opts) o = *opts;
+  else lsg_opts_default(&o);
+  int device = graphs[0].device;
+  std::vector<UnionMember> mem(ngraphs);
+  uint64_t V = 0, E = 0;
+  std::vector<uint32_t> seeds(ngraphs);
+  for (uint32_t i = 0; i < ngraphs; ++i) {
+    const lsg_graph &g = graphs[i];
+    if (!g.device_graph || g.device != device) return fail(LSG_ERR_ARG, "graph %u is not a device graph on device %d", i, device);
+    if (srcs[i] >= g.nnodes) return fail(LSG_ERR_ARG, "source of graph %u (%u) out of range [0, %u)", i, srcs[i], g.nnodes);
+    if (!dist_out[i]) return fail(LSG_ERR_ARG, "output %u is NULL", i);
+    mem[i] = UnionMember{g.row_start, g.edge_dst, (const uint32_t *)g.edge_data, dist_out[i], g.nnodes,
+                         (uint32_t)V, g.nedges, E};
+    seeds[i] = (uint32_t)V + srcs[i];
+    V += g.nnodes;
+    E += g.nedges;
+    if (V >= UINT32_MAX || E >= UINT32_MAX) return fail(LSG_ERR_ARG, "graphs exceed the u32 index range together");
+  }
+  Workspace *w;
+  int rc = get_workspace(device, &w);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(w->mu);
+  HIPCHK(hipSetDevice(device));
+  size_t mem_bytes = ngraphs * sizeof(UnionMember);
+  size_t bytes = (V + 1) * 4 + E * 8 + V * 4 + mem_bytes + 256;
+  void *ub;
+  if ((rc = ws_buffer(w, kBufUnion, bytes, &ub))) return rc;
+  uint32_t *urow = (uint32_t *)ub, *udst = urow + V + 1, *uw = udst + E, *udist = uw + E;
+  UnionMember *dmem = (UnionMember *)(((uintptr_t)(udist + V) + 255) & ~(uintptr_t)255);
+  HIPCHK(hipMemcpyAsync(dmem, mem.data(), mem_bytes, hipMemcpyHostToDevice, w->stream));
+  uint32_t blocks = (uint32_t)std::min<uint64_t>((E + V + 255) / 256 + 1, 16ull * w->cus);
+  hipLaunchKernelGGL(pack_union_kernel, dim3(blocks), dim3(256), 0, w->stream, dmem, ngraphs, urow, udst, uw,
+                     (uint32_t)V, E);
+  HIPCHK(hipGetLastError());
+  CsrView v{urow, udst, uw, (uint32_t)V, E};
+  lsg_stats st;
+  if ((rc = run_core<Algo::SSSP>(w, v, seeds.data(), ngraphs, o, udist, o.graph_launch != 0, &st))) return rc;
+  hipLaunchKernelGGL(unpack_union_kernel, dim3(blocks), dim3(256), 0, w->stream, dmem, ngraphs, udist, (uint32_t)V);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(w->stream));
+  if (stats) {
+    memset(stats, 0, sizeof *stats);
+    stats->ms_kernel = st.ms_kernel;
+    stats->rounds = st.near_iters;
+    stats->phases = st.iterations;
+    stats->items = st.items;
+    stats->heavy_chunks = st.heavy_chunks;
+    stats->grid_blocks = st.grid_blocks;
+    stats->block_size = st.block_size;
+    stats->launches = 1;
+    stats->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  return LSG_OK;
 }
 
 int lsg_last_trace(int device, lsg_round_trace *out, uint32_t cap, uint32_t *n, double *tick_hz) {

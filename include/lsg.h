@@ -87,7 +87,12 @@ typedef struct lsg_opts {
   int32_t heavy_deg;   /* degrees >= this are split into chunks of heavy_deg edges (default 4096) */
   int32_t sched;       /* 0 = fixed delta, 1 = learned per-phase delta (lsg_sched_set_model) */
   int32_t trace;       /* 1 = record one lsg_round_trace per round (lsg_last_trace) */
-  int32_t reserved[5];
+  int32_t graph_launch; /* 1 = launch the kernel sequence as a captured hipGraph (replayed while
+                           the arguments stay the same) */
+  int32_t pull_permille; /* batched: a round whose frontier holds at least this many per mille of
+                            the vertices gathers along in-edges (pull) instead of pushing along
+                            out-edges; 0 = always push */
+  int32_t reserved[3];
 } lsg_opts;
 
 /* One record per round of a single-source traversal, written when
@@ -193,6 +198,7 @@ typedef struct lsg_batch_stats {
   uint64_t rounds, phases, items, heavy_chunks;
   uint64_t lane_items;     /* (vertex, source) expansions */
   uint64_t edges_expanded; /* edge expansions, each shared by the sources of its vertex */
+  uint64_t improved_rows;  /* edge expansions in which at least one source improved */
   int32_t grid_blocks, block_size;
   int32_t launches;        /* persistent launches (one per 64 sources) */
   int32_t reserved[5];
@@ -208,6 +214,12 @@ int lsg_sssp_multi(const lsg_graph *g, const uint32_t *srcs, uint32_t nsrc, cons
  * device array of nnodes int32 for source i (host array of device pointers). */
 int lsg_sssp_batch(const lsg_graph *g, const uint32_t *srcs, uint32_t nsrc, const lsg_opts *opts,
                    int32_t *const *dist_out, lsg_batch_stats *stats);
+/* Many independent device graphs (same device), one source each, in ONE
+ * persistent launch: the graphs are packed into a disjoint union and run as
+ * one multi-seed traversal, which gives every graph exactly its own
+ * single-source distances.  dist_out[i] is a device array of graphs[i].nnodes. */
+int lsg_sssp_graphs(const lsg_graph *graphs, uint32_t ngraphs, const uint32_t *srcs, const lsg_opts *opts,
+                    int32_t *const *dist_out, lsg_batch_stats *stats);
 /* Per-phase feature records of the last traced batched call: nphases rows of
  * 20 floats (16 scheduler features, action or -1, delta, far mode, first round). */
 int lsg_last_features(int device, float *out, uint32_t cap_phases, uint32_t *nphases);
@@ -240,6 +252,9 @@ int lsg_output_dist(const int32_t *dist, uint32_t n, const char *path);
 int lsg_dev_malloc(int device, size_t bytes, void **ptr);
 int lsg_dev_free(int device, void *ptr);
 int lsg_dev_copy(int device, void *dst, const void *src, size_t bytes, int kind);
+/* Times a plain device-to-device copy of `bytes` (iters launches): the HBM
+ * reference figure, and the known-size kernel PMC counters are scaled by. */
+int lsg_stream_copy(int device, size_t bytes, int iters, double *ms_per_copy);
 
 /* Device synchronisation and name, for the CLI's "Using GPU:" line
  * (skelapp/skel.cu:146-148). */

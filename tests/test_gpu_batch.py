@@ -130,3 +130,50 @@ def test_mlp_train_and_infer_kernels():
     top2 = np.sort(logits, axis=1)[:, -2:]
     clear = (top2[:, 1] - top2[:, 0]) > 1e-3  # skip near-ties, where fp32 order may differ
     assert np.array_equal(pred[clear], np.argmax(logits, axis=1)[clear])
+
+
+def test_many_graphs_one_launch():
+    """Config 4B in small: graphs of uneven sizes, one source each, one persistent launch."""
+    from galoisgpu_amd.batch import sssp_graphs
+    rng = np.random.default_rng(21)
+    graphs, devs, srcs = [], [], []
+    for i in range(40):
+        g = CSRGraph.rmat(int(rng.integers(6, 13)), 8, seed=100 + i)
+        graphs.append(g)
+        devs.append(g.copy_to_gpu())
+        srcs.append(g.max_degree_vertex() if i % 3 else int(rng.integers(0, g.nnodes)))
+    tiny = CSRGraph.from_arrays([0, 0], [], [])  # a one-node graph among them
+    graphs.append(tiny)
+    devs.append(tiny.copy_to_gpu())
+    srcs.append(0)
+    for far, graph_launch in ((0, False), (1, False), (0, True)):
+        say("many graphs", far, graph_launch)
+        outs, st = sssp_graphs(devs, srcs, Options(delta=32, far_mode=far, graph_launch=graph_launch))
+        assert st["launches"] == 1
+        for g, s, d in zip(graphs, srcs, outs):
+            assert np.array_equal(d, oracle.dijkstra(g.row_start, g.edge_dst, g.edge_data, s))
+    for d in devs:
+        d.close()
+
+
+def test_graph_launch_replays_match_eager(rmat16):
+    """Captured hipGraph launches replayed several times, with eager work and
+    synchronisation in between, each equal to the eager result."""
+    from galoisgpu_amd.apps import sssp_into
+    from galoisgpu_amd.batch import DeviceArray, sssp_multi_into
+    g, srcs, ref = rmat16
+    dev = g.copy_to_gpu()
+    D = DeviceArray(g.nnodes * 64 * 4)
+    single = DeviceArray(g.nnodes * 4)
+    for rep in range(4):
+        say("graph replay", rep)
+        sssp_multi_into(dev, srcs, Options(delta=32, graph_launch=True), D)
+        assert np.array_equal(D.to_numpy(np.int32, (g.nnodes, 64)), ref), rep
+        # eager work between replays: a plain single-source run
+        sssp_into(dev, single.ptr, Options(delta=32, start_node=int(srcs[rep])))
+        assert np.array_equal(single.to_numpy(np.int32, g.nnodes), ref[:, rep])
+        sssp_into(dev, single.ptr, Options(delta=32, start_node=int(srcs[rep + 1]), graph_launch=True))
+        assert np.array_equal(single.to_numpy(np.int32, g.nnodes), ref[:, rep + 1])
+    D.close()
+    single.close()
+    dev.close()
