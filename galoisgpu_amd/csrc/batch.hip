@@ -102,9 +102,12 @@ __device__ __forceinline__ void expand_edges(const MArgs &a, const Sched &s, Lan
       myw = a.edge_data[e];
     }
     const uint32_t n = min((uint32_t)kWave, e_end - b);
-    bool pn = false, pf = false;
+    // Lane j collects the near / far source bits of edge j; the masks are
+    // then updated for all 64 edges with one vector atomic each.
+    uint64_t my_nb = 0, my_fb = 0;
     for (uint32_t j0 = 0; j0 < n; j0 += kUnroll) {
-      uint32_t v[kUnroll], nd[kUnroll], cur[kUnroll];
+      uint32_t v[kUnroll], nd[kUnroll], cur[kUnroll], old[kUnroll];
+      bool want[kUnroll];
 #pragma unroll
       for (int k = 0; k < kUnroll; ++k) {
         uint32_t j = min(j0 + k, n - 1);
@@ -112,23 +115,27 @@ __device__ __forceinline__ void expand_edges(const MArgs &a, const Sched &s, Lan
         nd[k] = du + readlane(myw, j);
         cur[k] = act ? ld_coh(a.D + (size_t)v[k] * a.S + lane) : 0u;
       }
+      // every atomicMin of the group is issued before any result is used
 #pragma unroll
       for (int k = 0; k < kUnroll; ++k) {
-        if (j0 + k >= n) break;
-        bool want = act && nd[k] < cur[k];
-        if (__ballot(want) == 0) continue;
-        uint32_t old = want ? atomic_min(a.D + (size_t)v[k] * a.S + lane, nd[k]) : 0u;
-        bool imp = want && nd[k] < old;
+        want[k] = act && j0 + k < n && nd[k] < cur[k];
+        old[k] = want[k] ? atomic_min(a.D + (size_t)v[k] * a.S + lane, nd[k]) : kInf;
+      }
+#pragma unroll
+      for (int k = 0; k < kUnroll; ++k) {
+        bool imp = want[k] && nd[k] < old[k];
         bool nearl = imp && (int64_t)nd[k] <= thr;
         if (imp && !nearl) ls.fmin = min(ls.fmin, nd[k]);
         uint64_t nb = __ballot(nearl), fb = __ballot(imp && !nearl);
         if (nb | fb) *improved_rows += 1;
         if (lane == j0 + k) {
-          if (nb) pn = atomic_or64(nm + v[k], nb) == 0;
-          if (fb) pf = atomic_or64(fm + v[k], fb) == 0;
+          my_nb = nb;
+          my_fb = fb;
         }
       }
     }
+    bool pn = my_nb != 0 && atomic_or64(nm + myv, my_nb) == 0;
+    bool pf = my_fb != 0 && atomic_or64(fm + myv, my_fb) == 0;
     wave_push(pn, myv, qn, &a.ctrl->qcount[s.near_q], a.qcap, &a.ctrl->abort);
     wave_push(pf, myv, qf, &a.ctrl->qcount[s.far_q], a.qcap, &a.ctrl->abort);
   }
