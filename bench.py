@@ -74,9 +74,9 @@ def main() -> int:
     launch_b = {k: v for k, v in tuned_b.items() if k in ("heavy_deg", "block_size")}
     launch_p = {k: v for k, v in tuned_p.items() if k in ("expand_mode", "block_size")}
     use_learned = False
-    model_path = lsched.default_model_path()
+    model_path = lsched.default_model_path("batched")
     if args.sched == "learned" or (args.sched == "auto" and os.path.exists(model_path)):
-        lsched.load_model(model_path, device=0)
+        lsched.load_model(model_path, device=0, kind="batched")
         use_learned = True
     # fixed batched schedule: the autotuner's choice (or the plain delta if never tuned)
     fixed_kw = dict(tuned_b) if tuned_b else dict(delta=args.delta)

@@ -28,7 +28,8 @@ ERRORS = {
     -8: "LSG_ERR_NODEV",
 }
 EXPAND_TWB, EXPAND_LB = 0, 1
-FAR_REFERENCE, FAR_SPLIT = 0, 1
+FAR_REFERENCE, FAR_SPLIT, FAR_BUCKET = 0, 1, 2
+SCHED_BATCHED, SCHED_PLAIN = 0, 1  # scheduler model kinds
 
 
 class LsgError(RuntimeError):
@@ -69,8 +70,7 @@ class LsgOpts(ctypes.Structure):
         ("sched", c_int32),
         ("trace", c_int32),
         ("graph_launch", c_int32),
-        ("pull_permille", c_int32),
-        ("reserved", c_int32 * 3),
+        ("reserved", c_int32 * 4),
     ]
 
 
@@ -169,7 +169,7 @@ SIGNATURES = {
                                 POINTER(LsgBatchStats)]),
     "lsg_last_features": (c_int, [c_int, c_void_p, c_uint32, POINTER(c_uint32)]),
     "lsg_sched_model_floats": (c_int, []),
-    "lsg_sched_set_model": (c_int, [c_int, c_void_p, c_uint32]),
+    "lsg_sched_set_model": (c_int, [c_int, c_int, c_void_p, c_uint32]),
     "lsg_mlp_train": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_uint32, c_uint32, c_float,
                               c_float, c_uint64, c_void_p]),
     "lsg_mlp_infer": (c_int, [c_int, c_void_p, c_void_p, c_uint32, c_void_p]),

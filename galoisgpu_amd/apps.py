@@ -33,7 +33,6 @@ class Options:
     sched: int = 0
     trace: bool = False
     graph_launch: bool = False
-    pull_permille: int = 0
 
     def to_c(self) -> LsgOpts:
         o = LsgOpts()

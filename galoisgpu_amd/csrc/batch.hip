@@ -32,13 +32,23 @@
 namespace lsg {
 
 #ifndef LSG_MULTI_UNROLL
-#define LSG_MULTI_UNROLL 4
+#define LSG_MULTI_UNROLL 8
 #endif
+#ifndef LSG_MULTI_ITEMS
+#define LSG_MULTI_ITEMS 8
+#endif
+// Waves per SIMD the register allocation must allow (0: compiler's choice).
+// 5 trades a few spills for a fifth resident block per CU (measured on
+// RMAT-22 x 64 sources: 72 ms at 5, 77 ms at the compiler's 4).
 #ifndef LSG_MULTI_WAVES
-#define LSG_MULTI_WAVES 0
+#define LSG_MULTI_WAVES 5
 #endif
 // Row loads issued ahead per wavefront in the edge loop.
 constexpr int kUnroll = LSG_MULTI_UNROLL;
+// Frontier items a wavefront takes per ticket: their masks, row bounds and
+// distance rows are loaded together and their edges relaxed as one list.
+constexpr int kItems = LSG_MULTI_ITEMS;
+static_assert(kItems >= 1 && kItems <= kWave, "items per ticket");
 
 struct MArgs {
   const uint32_t *row_start;
@@ -83,122 +93,174 @@ __device__ __forceinline__ uint64_t bcast64(uint64_t x, uint32_t l) {
   return ((uint64_t)hi << 32) | lo;
 }
 
-// Relaxes edges [e_begin, e_end) of u for the sources whose lanes hold a
-// finite du.  Row loads of four edges are issued before any is used.
-__device__ __forceinline__ void expand_edges(const MArgs &a, const Sched &s, LaneSched &ls, uint32_t du, bool act,
-                                             uint32_t e_begin, uint32_t e_end, uint64_t *improved_rows) {
+// Distance rows of the items a wavefront is expanding: dl[slot][lane] is the
+// item's distance for source `lane`, INF where that source does not expand it.
+// Each lane only reads its own column, so this is per-lane indexed storage.
+typedef uint32_t DistSlots[kItems][kWave];
+
+// Relaxes a batch of n <= 64 edges held one per lane: lane j has edge j's
+// destination ev, weight ew and the slot ek of the item it leaves.  The row
+// loads of a group of kUnroll edges are all issued before any is used, then
+// the group's atomicMins; lane j collects edge j's near / far source bits, and
+// the masks of the n destinations are updated with one vector atomic each.
+__device__ __forceinline__ void relax_batch(const MArgs &a, const Sched &s, LaneSched &ls, const DistSlots &dl,
+                                            uint32_t n, uint32_t ev, uint32_t ew, uint32_t ek,
+                                            uint64_t *improved_rows) {
   const uint32_t lane = lane_id();
   // Near/far threshold of this lane: the shared curdelta (apps/sssp/kernel.cu:134)
   // or, in a bucket phase, the lane's own source threshold.
   const int64_t thr = ls.bucket ? (int64_t)ls.thr : s.curdelta();
-  uint64_t *nm = a.nmask[s.round & 1];
-  uint64_t *fm = a.fmask[s.phase & 1];
-  uint32_t *qn = a.q[s.near_q], *qf = a.q[s.far_q];
-  for (uint32_t b = e_begin; b < e_end; b += kWave) {
-    uint32_t e = b + lane;
-    uint32_t myv = 0, myw = 0;
-    if (e < e_end) {
-      myv = a.edge_dst[e];
-      myw = a.edge_data[e];
+  uint64_t my_nb = 0, my_fb = 0;
+  for (uint32_t j0 = 0; j0 < n; j0 += kUnroll) {
+    uint32_t v[kUnroll], nd[kUnroll], cur[kUnroll], old[kUnroll];
+    bool want[kUnroll];
+#pragma unroll
+    for (int k = 0; k < kUnroll; ++k) {
+      const uint32_t j = min(j0 + k, (uint32_t)kWave - 1);
+      v[k] = readlane(ev, j);
+      const uint32_t du = j0 + k < n ? dl[readlane(ek, j)][lane] : kInf;
+      nd[k] = du == kInf ? kInf : du + readlane(ew, j);
+      cur[k] = nd[k] != kInf ? ld_coh(a.D + (size_t)v[k] * a.S + lane) : 0u;
     }
-    const uint32_t n = min((uint32_t)kWave, e_end - b);
-    // Lane j collects the near / far source bits of edge j; the masks are
-    // then updated for all 64 edges with one vector atomic each.
-    uint64_t my_nb = 0, my_fb = 0;
-    for (uint32_t j0 = 0; j0 < n; j0 += kUnroll) {
-      uint32_t v[kUnroll], nd[kUnroll], cur[kUnroll], old[kUnroll];
-      bool want[kUnroll];
+    // every atomicMin of the group is issued before any result is used
 #pragma unroll
-      for (int k = 0; k < kUnroll; ++k) {
-        uint32_t j = min(j0 + k, n - 1);
-        v[k] = readlane(myv, j);
-        nd[k] = du + readlane(myw, j);
-        cur[k] = act ? ld_coh(a.D + (size_t)v[k] * a.S + lane) : 0u;
-      }
-      // every atomicMin of the group is issued before any result is used
+    for (int k = 0; k < kUnroll; ++k) {
+      want[k] = nd[k] < cur[k];
+      old[k] = want[k] ? atomic_min(a.D + (size_t)v[k] * a.S + lane, nd[k]) : kInf;
+    }
 #pragma unroll
-      for (int k = 0; k < kUnroll; ++k) {
-        want[k] = act && j0 + k < n && nd[k] < cur[k];
-        old[k] = want[k] ? atomic_min(a.D + (size_t)v[k] * a.S + lane, nd[k]) : kInf;
-      }
-#pragma unroll
-      for (int k = 0; k < kUnroll; ++k) {
-        bool imp = want[k] && nd[k] < old[k];
-        bool nearl = imp && (int64_t)nd[k] <= thr;
-        if (imp && !nearl) ls.fmin = min(ls.fmin, nd[k]);
-        uint64_t nb = __ballot(nearl), fb = __ballot(imp && !nearl);
-        if (nb | fb) *improved_rows += 1;
-        if (lane == j0 + k) {
-          my_nb = nb;
-          my_fb = fb;
-        }
+    for (int k = 0; k < kUnroll; ++k) {
+      bool imp = want[k] && nd[k] < old[k];
+      bool nearl = imp && (int64_t)nd[k] <= thr;
+      if (imp && !nearl) ls.fmin = min(ls.fmin, nd[k]);
+      uint64_t nb = __ballot(nearl), fb = __ballot(imp && !nearl);
+      if (nb | fb) *improved_rows += 1;
+      if (lane == j0 + k) {
+        my_nb = nb;
+        my_fb = fb;
       }
     }
-    bool pn = my_nb != 0 && atomic_or64(nm + myv, my_nb) == 0;
-    bool pf = my_fb != 0 && atomic_or64(fm + myv, my_fb) == 0;
-    wave_push(pn, myv, qn, &a.ctrl->qcount[s.near_q], a.qcap, &a.ctrl->abort);
-    wave_push(pf, myv, qf, &a.ctrl->qcount[s.far_q], a.qcap, &a.ctrl->abort);
   }
+  // Two edges of a batch may share a destination; only the lane whose OR
+  // finds the mask empty queues it.
+  bool pn = my_nb != 0 && atomic_or64(a.nmask[s.round & 1] + ev, my_nb) == 0;
+  bool pf = my_fb != 0 && atomic_or64(a.fmask[s.phase & 1] + ev, my_fb) == 0;
+  wave_push(pn, ev, a.q[s.near_q], &a.ctrl->qcount[s.near_q], a.qcap, &a.ctrl->abort);
+  wave_push(pf, ev, a.q[s.far_q], &a.ctrl->qcount[s.far_q], a.qcap, &a.ctrl->abort);
 }
 
-// One frontier item: consume its mask, load its distance row, and expand it
-// (or queue it as heavy chunks for the next round).
-__device__ __forceinline__ void process_item(const MArgs &a, const Sched &s, LaneSched &ls, uint32_t i,
-                                             MultiCounters &mc) {
+// Queues heavy vertex u (edges [rs, re)) as chunks of heavy_deg edges, each
+// expanded by a wavefront in the next round with the sources active now.
+__device__ __forceinline__ void queue_heavy(const MArgs &a, const Sched &s, uint32_t u, uint32_t rs, uint32_t re,
+                                            uint64_t am) {
   const uint32_t lane = lane_id();
-  uint32_t u = ld_coh(a.q[s.in_q] + i);
-  if (u >= a.nnodes) return;
-  uint64_t *mp = s.aux[0] ? a.fmask[(s.phase - 1) & 1] + u : a.nmask[(s.round - 1) & 1] + u;
-  uint64_t m = 0;
-  if (lane == 0) m = atomic_exch64(mp, 0ull);
-  m = bcast64(m, 0);
-  if (m == 0) return;
-  bool act = lane < a.S && ((m >> lane) & 1);
-  uint32_t du = act ? ld_coh(a.D + (size_t)u * a.S + lane) : kInf;
-  act = act && du != kInf;
-  if (s.aux[0] && s.aux[1]) {
-    // Split / bucket far pile: a source whose distance is still above this
-    // phase's threshold (shared, or its own in a bucket phase) waits for a
-    // later phase instead of expanding early.
-    bool defer = act && (ls.bucket ? du > ls.thr : (int64_t)du > s.curdelta());
-    uint64_t dm = __ballot(defer);
-    if (dm) {
-      bool push = false;
-      if (lane == 0) push = atomic_or64(a.fmask[s.phase & 1] + u, dm) == 0;
-      wave_push(push, u, a.q[s.far_q], &a.ctrl->qcount[s.far_q], a.qcap, &a.ctrl->abort);
-      if (defer) ls.fmin = min(ls.fmin, du);
-      act = act && !defer;
+  uint32_t nch = (re - rs + a.heavy_deg - 1) / a.heavy_deg;
+  uint32_t hqi = 1u - s.heavy_in;
+  uint32_t base = 0;
+  if (lane == 0) base = atomic_add(&a.ctrl->hcount[hqi], nch);
+  base = readlane(base, 0);
+  for (uint32_t k = lane; k < nch; k += kWave) {
+    uint32_t pos = base + k;
+    if (pos < a.hcap) {
+      st_coh64(a.hq[hqi] + 2ull * pos, ((uint64_t)k << 32) | u);
+      st_coh64(a.hq[hqi] + 2ull * pos + 1, am);
+    } else {
+      st_coh(&a.ctrl->abort, kAbortCapacity);
     }
   }
-  uint64_t am = __ballot(act);
-  if (am == 0) return;
-  uint32_t rs = a.row_start[u], re = a.row_start[u + 1];
-  if (re - rs >= a.heavy_deg) {
-    // Heavy vertex: chunks of heavy_deg edges, each expanded by a wavefront
-    // in the next round with the sources active now.
-    uint32_t nch = (re - rs + a.heavy_deg - 1) / a.heavy_deg;
-    uint32_t hqi = 1u - s.heavy_in;
-    uint32_t base = 0;
-    if (lane == 0) base = atomic_add(&a.ctrl->hcount[hqi], nch);
-    base = readlane(base, 0);
-    for (uint32_t k = lane; k < nch; k += kWave) {
-      uint32_t pos = base + k;
-      if (pos < a.hcap) {
-        st_coh64(a.hq[hqi] + 2ull * pos, ((uint64_t)k << 32) | u);
-        st_coh64(a.hq[hqi] + 2ull * pos + 1, am);
-      } else {
-        st_coh(&a.ctrl->abort, kAbortCapacity);
-      }
-    }
-    return;
-  }
-  mc.lane_items += __builtin_popcountll(am);
-  mc.edges += re - rs;
-  expand_edges(a, s, ls, du, act, rs, re, &mc.improved);
 }
 
+// Frontier items [base, base + cnt), cnt <= kItems: lane k consumes item k's
+// mask and loads its row bounds; the distance rows of all the items are then
+// loaded together and parked in dl, and the items' edges are relaxed as one
+// flattened list, 64 per batch.  Heavy items are queued as chunks instead.
+__device__ __forceinline__ void process_items(const MArgs &a, const Sched &s, LaneSched &ls, uint32_t base,
+                                              DistSlots &dl, MultiCounters &mc) {
+  const uint32_t lane = lane_id();
+  const uint32_t cnt = min((uint32_t)kItems, s.in_count - base);
+  uint64_t *mp = s.aux[0] ? a.fmask[(s.phase - 1) & 1] : a.nmask[(s.round - 1) & 1];
+  uint32_t u = a.nnodes;
+  if (lane < cnt) u = ld_coh(a.q[s.in_q] + base + lane);
+  uint64_t m = 0;
+  uint32_t rs = 0, re = 0;
+  if (u < a.nnodes) {
+    m = atomic_exch64(mp + u, 0ull);
+    rs = a.row_start[u];
+    re = a.row_start[u + 1];
+  }
+  // The row loads depend on the exchanged masks, so each reads a distance at
+  // least as new as the improvements that set the consumed bits.
+  uint32_t du[kItems];
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    const uint64_t mk = bcast64(m, k);
+    const uint32_t uk = readlane(u, k);
+    du[k] = lane < a.S && ((mk >> lane) & 1) ? ld_coh(a.D + (size_t)uk * a.S + lane) : kInf;
+  }
+  const bool split = s.aux[0] && s.aux[1];
+  uint32_t mydeg = 0;  // lane k: edges of item k to relax in this wavefront
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    bool act = du[k] != kInf;
+    if (__ballot(act) == 0) continue;
+    if (split) {
+      // Split / bucket far pile: a source whose distance is still above this
+      // phase's threshold (shared, or its own in a bucket phase) waits for a
+      // later phase instead of expanding early.
+      const bool defer = act && (ls.bucket ? du[k] > ls.thr : (int64_t)du[k] > s.curdelta());
+      const uint64_t dm = __ballot(defer);
+      if (dm) {
+        const uint32_t uk = readlane(u, k);
+        bool push = false;
+        if (lane == 0) push = atomic_or64(a.fmask[s.phase & 1] + uk, dm) == 0;
+        wave_push(push, uk, a.q[s.far_q], &a.ctrl->qcount[s.far_q], a.qcap, &a.ctrl->abort);
+        if (defer) ls.fmin = min(ls.fmin, du[k]);
+        act = act && !defer;
+      }
+    }
+    const uint64_t am = __ballot(act);
+    if (am == 0) continue;
+    const uint32_t rsk = readlane(rs, k), rek = readlane(re, k);
+    if (rek - rsk >= a.heavy_deg) {
+      queue_heavy(a, s, readlane(u, k), rsk, rek, am);
+      continue;
+    }
+    if (rek == rsk) continue;
+    dl[k][lane] = act ? du[k] : kInf;
+    mc.lane_items += __builtin_popcountll(am);
+    mc.edges += rek - rsk;
+    if (lane == (uint32_t)k) mydeg = rek - rsk;
+  }
+  // Inclusive prefix of the items' edge counts over lanes 0..kItems-1: item k
+  // owns flattened edges [end_k - deg_k, end_k).
+  uint32_t end = mydeg;
+#pragma unroll
+  for (int d = 1; d < kItems; d <<= 1) {
+    const uint32_t t = __shfl_up(end, d);
+    if (lane >= (uint32_t)d) end += t;
+  }
+  const uint32_t total = readlane(end, kItems - 1);
+  const uint32_t first = end - mydeg;
+  for (uint32_t b0 = 0; b0 < total; b0 += kWave) {
+    const uint32_t f = b0 + lane;
+    // the item holding flattened edge f is the number of items ending at or before f
+    uint32_t item = 0;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) item += readlane(end, k) <= f;
+    const uint32_t e = __shfl(rs, item) + (f - __shfl(first, item));
+    uint32_t ev = 0, ew = 0;
+    if (f < total) {
+      ev = a.edge_dst[e];
+      ew = a.edge_data[e];
+    }
+    relax_batch(a, s, ls, dl, min((uint32_t)kWave, total - b0), ev, ew, item, &mc.improved);
+  }
+}
+
+// Heavy chunk c: edges [k * heavy_deg, (k + 1) * heavy_deg) of u, for the
+// sources recorded with the chunk; the distance row goes to slot 0.
 __device__ __forceinline__ void process_chunk(const MArgs &a, const Sched &s, LaneSched &ls, uint32_t c,
-                                              MultiCounters &mc) {
+                                              DistSlots &dl, MultiCounters &mc) {
   const uint32_t lane = lane_id();
   uint64_t ent = ld_coh64(a.hq[s.heavy_in] + 2ull * c);
   uint64_t m = ld_coh64(a.hq[s.heavy_in] + 2ull * c + 1);
@@ -214,7 +276,15 @@ __device__ __forceinline__ void process_chunk(const MArgs &a, const Sched &s, La
   if (am == 0) return;
   mc.lane_items += __builtin_popcountll(am);
   mc.edges += hi - lo;
-  expand_edges(a, s, ls, du, act, lo, hi, &mc.improved);
+  dl[0][lane] = act ? du : kInf;
+  for (uint32_t b = lo; b < hi; b += kWave) {
+    uint32_t ev = 0, ew = 0;
+    if (b + lane < hi) {
+      ev = a.edge_dst[b + lane];
+      ew = a.edge_data[b + lane];
+    }
+    relax_batch(a, s, ls, dl, min((uint32_t)kWave, hi - b), ev, ew, 0u, &mc.improved);
+  }
 }
 
 __device__ __forceinline__ void stat_add64(uint64_t *p, uint64_t v) { st_coh64(p, ld_coh64(p) + v); }
@@ -233,6 +303,9 @@ __device__ void multi_leader(const MArgs &a, MultiCounters *tot) {
   const uint32_t t = threadIdx.x;
   if (t == 0) {
     sched_load(s, &c->s);
+    // every wavefront has left the round's item loops: restart the tickets
+    st_coh(&c->ticket, 0u);
+    st_coh(&c->ticket2, 0u);
     uint32_t nnear = ld_coh(&c->qcount[s.near_q]);
     uint32_t nfar = ld_coh(&c->qcount[s.far_q]);
     uint32_t nheavy = ld_coh(&c->hcount[1u - s.heavy_in]);
@@ -345,6 +418,7 @@ void multi_kernel(MArgs a, MultiCounters *tot) {
   __shared__ Sched ss;
   __shared__ uint32_t quit;
   __shared__ unsigned long long blk[3];
+  __shared__ DistSlots dls[BS / kWave];
   Ctrl *c = a.ctrl;
   const uint32_t nb = gridDim.x;
   if (!residency_handshake(c, nb, a.residency_limit)) return;
@@ -367,8 +441,22 @@ void multi_kernel(MArgs a, MultiCounters *tot) {
     ls.bucket = s.aux[1] == LSG_FAR_BUCKET;
     ls.thr = ls.bucket && lane_id() < a.S ? ld_coh(a.thr + lane_id()) : 0u;
     ls.fmin = kInf;
-    for (uint32_t i = wave; i < s.in_count; i += nwaves) process_item(a, s, ls, i, mc);
-    for (uint32_t k = wave; k < s.heavy_count; k += nwaves) process_chunk(a, s, ls, k, mc);
+    DistSlots &dl = dls[threadIdx.x / kWave];
+    // Static first pass (ticket = wavefront id), then dynamic tickets; the
+    // next ticket is fetched while the current one is processed.
+    const uint32_t nchunks = (s.in_count + kItems - 1) / kItems;
+    for (uint32_t ck = wave; ck < nchunks;) {
+      uint32_t t = 0;
+      if (lane_id() == 0) t = atomic_add(&c->ticket, 1u);
+      process_items(a, s, ls, ck * kItems, dl, mc);
+      ck = nwaves + readlane(t, 0);
+    }
+    for (uint32_t ck = wave; ck < s.heavy_count;) {
+      uint32_t t = 0;
+      if (lane_id() == 0) t = atomic_add(&c->ticket2, 1u);
+      process_chunk(a, s, ls, ck, dl, mc);
+      ck = nwaves + readlane(t, 0);
+    }
     // this wavefront's smallest far push per source -> phase minimum
     if (__ballot(ls.fmin != kInf) && ls.fmin != kInf) atomic_min(a.fmin + (s.phase & 1) * 64 + lane_id(), ls.fmin);
     if (lane_id() == 0 && (mc.lane_items | mc.edges)) {
@@ -423,6 +511,7 @@ __global__ void multi_seed(MArgs a) {
     c->hcount[0] = 0;
     c->hcount[1] = 0;
     c->ticket = 0;
+    c->ticket2 = 0;
     c->abort = kAbortNone;
     c->checkin = 0;
     c->resident_seen = 0;
@@ -516,8 +605,9 @@ static int run_multi(Workspace *w, const lsg_graph *g, const uint32_t *srcs_host
   a.thr = (uint32_t *)((char *)sbuf + 1024);
   if (o.sched) {
     a.sched = 1;
-    a.mlp = sched_model_device(w->device);
-    if (!a.mlp) return fail(LSG_ERR_ARG, "learned scheduler requested but no model is loaded (lsg_sched_set_model)");
+    a.mlp = sched_model_device(w->device, LSG_SCHED_BATCHED);
+    if (!a.mlp)
+      return fail(LSG_ERR_ARG, "learned scheduler requested but no batched-kernel model is loaded (lsg_sched_set_model)");
   }
   if (o.far_mode != LSG_FAR_REFERENCE && o.far_mode != LSG_FAR_SPLIT && o.far_mode != LSG_FAR_BUCKET)
     return fail(LSG_ERR_ARG, "bad far_mode %d", o.far_mode);

@@ -112,11 +112,11 @@ struct Ctrl {
   uint32_t bar_gen;   uint32_t _p1[31];
   uint32_t qcount[3]; uint32_t _p2[29];
   uint32_t hcount[2]; uint32_t _p3[30];
-  uint32_t ticket;    uint32_t _p4[31];
+  uint32_t ticket;    uint32_t ticket2; uint32_t _p4[30];
   uint32_t abort;     uint32_t checkin; uint32_t resident_seen; uint32_t _p5[29];
   Sched s;            uint32_t _p6[16];
-  uint64_t stat_items, stat_rounds, stat_heavy, stat_phases;
-  uint32_t _p7[24];
+  uint64_t stat_items, stat_rounds, stat_heavy, stat_phases, stat_edges;
+  uint32_t _p7[22];
 };
 
 // ---- wave64 helpers --------------------------------------------------------

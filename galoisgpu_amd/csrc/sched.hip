@@ -16,13 +16,18 @@
 
 namespace lsg {
 
+// Loaded models, keyed by (device, kind); each lives until the process ends.
+struct ModelSlot {
+  int device, kind;
+  float *dptr;
+};
 static std::mutex g_model_mu;
-static std::vector<std::pair<int, float *>> g_models;
+static std::vector<ModelSlot> g_models;
 
-const float *sched_model_device(int device) {
+const float *sched_model_device(int device, int kind) {
   std::lock_guard<std::mutex> lk(g_model_mu);
   for (auto &e : g_models)
-    if (e.first == device) return e.second;
+    if (e.device == device && e.kind == kind) return e.dptr;
   return nullptr;
 }
 
@@ -208,7 +213,8 @@ extern "C" {
 
 int lsg_sched_model_floats(void) { return kModelFloats; }
 
-int lsg_sched_set_model(int device, const float *model, uint32_t nfloats) {
+int lsg_sched_set_model(int device, int kind, const float *model, uint32_t nfloats) {
+  if (kind != LSG_SCHED_BATCHED && kind != LSG_SCHED_PLAIN) return fail(LSG_ERR_ARG, "bad scheduler model kind %d", kind);
   if (!model || nfloats != (uint32_t)kModelFloats)
     return fail(LSG_ERR_ARG, "scheduler model must have %d floats (got %u)", kModelFloats, nfloats);
   for (uint32_t i = 0; i < nfloats; ++i)
@@ -220,10 +226,10 @@ int lsg_sched_set_model(int device, const float *model, uint32_t nfloats) {
   std::lock_guard<std::mutex> lk(g_model_mu);
   float *dptr = nullptr;
   for (auto &e : g_models)
-    if (e.first == device) dptr = e.second;
+    if (e.device == device && e.kind == kind) dptr = e.dptr;
   if (!dptr) {
     HIPCHK(hipMalloc(&dptr, kModelFloats * sizeof(float)));
-    g_models.push_back({device, dptr});
+    g_models.push_back({device, kind, dptr});
   }
   HIPCHK(hipMemcpy(dptr, model, kModelFloats * sizeof(float), hipMemcpyHostToDevice));
   return LSG_OK;

@@ -26,7 +26,9 @@ constexpr int kOffB2 = kOffW2 + kHid * kHid;
 constexpr int kOffW3 = kOffB2 + kHid;         // [kHid][kAct]
 constexpr int kOffB3 = kOffW3 + kHid * kAct;
 constexpr int kOffDeltaMult = kOffB3 + kAct;  // delta multiplier per action
-constexpr int kOffFarMode = kOffDeltaMult + kAct;    // far-pile mode per action (0 reference, 1 split)
+// Mode per action: the far-pile mode (LSG_FAR_*) in a batched-kernel model,
+// the expansion mode (LSG_EXPAND_*) in a plain-kernel model.
+constexpr int kOffFarMode = kOffDeltaMult + kAct;
 constexpr int kOffForced = kOffFarMode + kAct;       // forced action per phase (< 0: use the MLP)
 constexpr int kModelFloats = kOffForced + kMaxForced;
 // Per-phase feature record: 16 features, action (-1 fixed), delta, far mode,
@@ -106,17 +108,19 @@ struct MlpScratch {
 };
 
 // Forward pass spread over the block (needs >= kHid threads; every thread of
-// the block must call it).  Uses the forced schedule when one is set for
-// f.phase.  Returns the action in every thread.
+// the block must call it).  Uses the forced schedule when one is set: entry
+// min(phase, kMaxForced - 1), so the last entry holds for every later phase.
+// Returns the action in every thread.
 __device__ inline int mlp_block_action(const float *m, const SchedFeatures &f, MlpScratch &sc) {
   const int t = threadIdx.x;
-  bool forced = f.phase < (uint32_t)kMaxForced && m[kOffForced + f.phase] >= 0.0f;
+  const uint32_t fp = f.phase < (uint32_t)kMaxForced ? f.phase : (uint32_t)kMaxForced - 1;
+  bool forced = m[kOffForced + fp] >= 0.0f;
   if (t == 0) {
     float x[kFeat];
     sched_featurize(f, x);
 #pragma unroll 1
     for (int i = 0; i < kFeat; ++i) sc.x[i] = (x[i] - m[kOffMean + i]) * m[kOffInvStd + i];
-    sc.action = forced ? (int)m[kOffForced + f.phase] : -1;
+    sc.action = forced ? (int)m[kOffForced + fp] : -1;
   }
   __syncthreads();
   if (!forced) {
@@ -160,6 +164,7 @@ __host__ __device__ inline uint32_t action_delta(const float *m, int act, uint32
 }
 
 // Device copy of the scheduler model for `device`, or null if none is set.
-const float *sched_model_device(int device);
+// Device copy of the model of `kind` (LSG_SCHED_*) loaded on `device`, or null.
+const float *sched_model_device(int device, int kind);
 
 }  // namespace lsg

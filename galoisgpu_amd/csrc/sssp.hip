@@ -27,6 +27,7 @@
 #include "engine.h"
 #include "lsg_internal.h"
 #include "runtime.h"
+#include "sched_mlp.h"
 
 namespace lsg {
 
@@ -56,6 +57,10 @@ struct KArgs {
   uint64_t residency_limit;  // start-up handshake bound, wall-clock ticks
   lsg_round_trace *trace;    // per-round records, or null
   uint32_t trace_cap;
+  uint32_t sched;            // 1 = learned delta width and expansion mode per phase
+  const float *mlp;          // scheduler weights (plain-kernel model), or null
+  float *feat;               // per-phase feature records [kMaxForced][kFeatRec], or null
+  uint64_t nedges;
 };
 
 // ---- per-edge operators ---------------------------------------------------
@@ -162,6 +167,7 @@ struct TileShared {
   uint32_t start[BS];
   uint32_t du[BS];
   uint32_t wsum[BS / 64 + 1];
+  unsigned long long edges;  // edges expanded by this block in the current round
 };
 
 // One tile of the in-queue: items [t*BS, t*BS+BS).  Loads the row bounds of
@@ -209,8 +215,16 @@ __device__ void process_tile(const KArgs &a, const Sched &s, uint32_t t, TileSha
     }
   }
   if (heavy) deg = 0;
+  if (a.sched || a.feat) {
+    // edges expanded, one of the scheduler's features
+    uint32_t ws = deg;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) ws += __shfl_xor(ws, off);
+    if (lane_id() == 0 && ws) atomicAdd(&sh.edges, (unsigned long long)ws);
+  }
 
-  if (a.expand_mode == LSG_EXPAND_LB) {
+  // Expansion mode of this phase (the learned scheduler may change it).
+  if (s.aux[2] == LSG_EXPAND_LB) {
     // Tile-flattened expansion: prefix sum of degrees, then every lane takes
     // one edge of the flattened tile (the reference's fg scheduler,
     // apps/sssp/kernel.cu:266-302, at block granularity).
@@ -262,7 +276,7 @@ __device__ void process_tile(const KArgs &a, const Sched &s, uint32_t t, TileSha
 // One heavy chunk: heavy_deg consecutive edges of one vertex, spread over
 // the block (replaces the -l edge-balanced TB_LB pass, apps/sssp/kernel.cu:78-149).
 template <Algo A, int BS>
-__device__ void process_heavy(const KArgs &a, const Sched &s, uint32_t c) {
+__device__ void process_heavy(const KArgs &a, const Sched &s, uint32_t c, TileShared<BS> &sh) {
   uint64_t ent = ld_coh64(reinterpret_cast<const uint64_t *>(a.hq[s.heavy_in]) + c);
   uint32_t u = (uint32_t)ent, k = (uint32_t)(ent >> 32);
   if (u >= a.nnodes) return;
@@ -271,78 +285,119 @@ __device__ void process_heavy(const KArgs &a, const Sched &s, uint32_t c) {
   if ((uint64_t)k * a.heavy_deg >= re - rs) return;
   uint32_t lo = rs + k * a.heavy_deg;
   uint32_t hi = min(re, lo + a.heavy_deg);
+  if ((a.sched || a.feat) && threadIdx.x == 0) atomicAdd(&sh.edges, (unsigned long long)(hi - lo));
   for (uint32_t b = lo; b < hi; b += BS) {
     uint32_t e = b + threadIdx.x;
     relax_edge<A>(a, s, e, du, e < hi);
   }
 }
 
-// Leader step between rounds (single thread).  Implements the pile rotation of
-// the persistent driver (apps/sssp/kernel.cu:363-384): iterate the near pile to
-// empty (retry2), then make the far pile the next phase's input (advance2)
-// with curdelta += DELTA; finish when both are empty.
 __device__ __forceinline__ void stat_add(uint64_t *p, uint64_t v) { st_coh64(p, ld_coh64(p) + v); }
 
+// Leader step between rounds, run by every thread of the last block to reach
+// the barrier.  Implements the pile rotation of the persistent driver
+// (apps/sssp/kernel.cu:363-384): iterate the near pile to empty (retry2), then
+// make the far pile the next phase's input (advance2) with curdelta += DELTA;
+// finish when both are empty.  At a phase boundary the learned scheduler (when
+// enabled) picks the next phase's delta width and expansion mode; it never
+// changes which vertices are relaxed or how, only when.
 template <Algo A>
 __device__ void leader_step(const KArgs &a) {
+  __shared__ Sched s;
+  __shared__ SchedFeatures f;
+  __shared__ MlpScratch sc;
+  __shared__ uint32_t decide, consumed;
   Ctrl *c = a.ctrl;
-  Sched s;
-  sched_load(s, &c->s);
-  uint32_t nnear = ld_coh(&c->qcount[s.near_q]);
-  uint32_t nfar = ld_coh(&c->qcount[s.far_q]);
-  uint32_t nheavy = ld_coh(&c->hcount[1u - s.heavy_in]);
-  if (nnear > a.qcap || nfar > a.qcap || nheavy > a.hcap) {
-    // A push found its queue full (and raised kAbortCapacity); never hand
-    // out a count beyond the buffer.
-    st_coh(&c->abort, kAbortCapacity);
-    s.done = 1;
+  if (threadIdx.x == 0) {
+    decide = 0;
+    sched_load(s, &c->s);
+    uint32_t nnear = ld_coh(&c->qcount[s.near_q]);
+    uint32_t nfar = ld_coh(&c->qcount[s.far_q]);
+    uint32_t nheavy = ld_coh(&c->hcount[1u - s.heavy_in]);
+    if (nnear > a.qcap || nfar > a.qcap || nheavy > a.hcap) {
+      // A push found its queue full (and raised kAbortCapacity); never hand
+      // out a count beyond the buffer.
+      st_coh(&c->abort, kAbortCapacity);
+      s.done = 1;
+    } else {
+      stat_add(&c->stat_items, s.in_count);
+      stat_add(&c->stat_heavy, s.heavy_count);
+      stat_add(&c->stat_rounds, 1);
+      if (a.trace && s.round - 1 < a.trace_cap) {
+        lsg_round_trace &r = a.trace[s.round - 1];
+        r.round = s.round;
+        r.phase = s.phase;
+        r.in_items = s.in_count;
+        r.heavy_chunks = s.heavy_count;
+        r.near_out = nnear;
+        r.far_out = nfar;
+        r.heavy_out = nheavy;
+        r.curdelta_lo = s.curdelta_lo;
+        r.curdelta_hi = s.curdelta_hi;
+        r.delta = s.delta;
+        r.t_end = wall_clock64();
+      }
+      consumed = s.in_q;
+      st_coh(&c->hcount[s.heavy_in], 0u);  // consumed heavy queue becomes the next producer
+      s.heavy_in = 1u - s.heavy_in;
+      s.heavy_count = nheavy;
+      if (nnear > 0 || nheavy > 0) {
+        s.in_q = s.near_q;
+        s.in_count = nnear;
+        s.near_q = consumed;
+        st_coh(&c->qcount[consumed], 0u);
+        s.aux[0] = 0;
+        s.round += 1;
+      } else if (A == Algo::SSSP && nfar > 0) {
+        decide = 1;
+        s.in_count = nfar;
+        f.in_items = nfar;
+        f.phase = s.phase;
+        f.round = s.round;
+        f.curdelta = (float)s.curdelta();
+        f.base_delta = a.delta;
+        f.prev_delta = s.delta;
+        f.lane_items = (float)ld_coh64(&c->stat_items);
+        f.edges = (float)ld_coh64(&c->stat_edges);
+        f.nnodes = a.nnodes;
+        f.nedges = (float)a.nedges;
+        f.nsrc = 1;
+      } else {
+        s.done = 1;
+        stat_add(&c->stat_phases, 1);
+      }
+    }
+  }
+  __syncthreads();
+  if (decide) {
+    int act = a.sched ? mlp_block_action(a.mlp, f, sc) : -1;
+    if (threadIdx.x == 0) {
+      if (act >= 0) {
+        s.delta = action_delta(a.mlp, act, a.delta);
+        s.aux[2] = (uint32_t)a.mlp[kOffFarMode + act];  // plain-kernel models: expansion mode
+      }
+      if (a.feat && s.phase < (uint32_t)kMaxForced) {
+        float *rec = a.feat + (size_t)s.phase * kFeatRec;
+        sched_featurize(f, rec);
+        rec[kFeat] = (float)act;
+        rec[kFeat + 1] = (float)s.delta;
+        rec[kFeat + 2] = (float)s.aux[2];
+        rec[kFeat + 3] = (float)(s.round + 1);  // first round of the phase being started
+      }
+      s.in_q = s.far_q;
+      s.far_q = consumed;
+      st_coh(&c->qcount[consumed], 0u);
+      s.aux[0] = 1;  // this round pops the far pile
+      s.set_curdelta(s.curdelta() + s.delta);
+      s.phase += 1;
+      s.round += 1;
+      stat_add(&c->stat_phases, 1);
+    }
+  }
+  if (threadIdx.x == 0) {
     sched_store(&c->s, s);
-    return;
+    st_coh(&c->ticket, 0u);
   }
-  stat_add(&c->stat_items, s.in_count);
-  stat_add(&c->stat_heavy, s.heavy_count);
-  stat_add(&c->stat_rounds, 1);
-  if (a.trace && s.round - 1 < a.trace_cap) {
-    lsg_round_trace &r = a.trace[s.round - 1];
-    r.round = s.round;
-    r.phase = s.phase;
-    r.in_items = s.in_count;
-    r.heavy_chunks = s.heavy_count;
-    r.near_out = nnear;
-    r.far_out = nfar;
-    r.heavy_out = nheavy;
-    r.curdelta_lo = s.curdelta_lo;
-    r.curdelta_hi = s.curdelta_hi;
-    r.delta = s.delta;
-    r.t_end = wall_clock64();
-  }
-  uint32_t consumed = s.in_q;
-  st_coh(&c->hcount[s.heavy_in], 0u);  // consumed heavy queue becomes the next producer
-  s.heavy_in = 1u - s.heavy_in;
-  s.heavy_count = nheavy;
-  if (nnear > 0 || nheavy > 0) {
-    s.in_q = s.near_q;
-    s.in_count = nnear;
-    s.near_q = consumed;
-    st_coh(&c->qcount[consumed], 0u);
-    s.aux[0] = 0;
-    s.round += 1;
-  } else if (A == Algo::SSSP && nfar > 0) {
-    s.in_q = s.far_q;
-    s.in_count = nfar;
-    s.far_q = consumed;
-    st_coh(&c->qcount[consumed], 0u);
-    s.aux[0] = 1;  // this round pops the far pile
-    s.set_curdelta(s.curdelta() + s.delta);
-    s.phase += 1;
-    s.round += 1;
-    stat_add(&c->stat_phases, 1);
-  } else {
-    s.done = 1;
-    stat_add(&c->stat_phases, 1);
-  }
-  sched_store(&c->s, s);
-  st_coh(&c->ticket, 0u);
 }
 
 template <Algo A, int BS>
@@ -360,6 +415,7 @@ __global__ __launch_bounds__(BS) void persistent_kernel(KArgs a) {
       sched_load(s, &c->s);
       ss = s;
       quit = s.done || ld_coh(&c->abort) != kAbortNone;
+      sh.edges = 0;
     }
     __syncthreads();
     if (quit) break;
@@ -370,14 +426,19 @@ __global__ __launch_bounds__(BS) void persistent_kernel(KArgs a) {
     uint32_t t = blockIdx.x;
     while (t < total) {
       if (t < ntiles) process_tile<A, BS>(a, s, t, sh);
-      else process_heavy<A, BS>(a, s, t - ntiles);
+      else process_heavy<A, BS>(a, s, t - ntiles, sh);
       if (total <= nb) break;
       __syncthreads();
       if (threadIdx.x == 0) tk = nb + atomic_add(&c->ticket, 1u);
       __syncthreads();
       t = tk;
     }
-    grid_barrier(c, nb, a.spin_limit, [&] { leader_step<A>(a); });
+    if (a.sched || a.feat) {
+      __syncthreads();
+      if (threadIdx.x == 0 && sh.edges)
+        __hip_atomic_fetch_add(&c->stat_edges, (uint64_t)sh.edges, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    grid_barrier_block(c, nb, a.spin_limit, [&] { leader_step<A>(a); });
   }
 }
 
@@ -412,8 +473,9 @@ __global__ void init_kernel(KArgs a, uint32_t delta) {
     s.set_curdelta(0);  // apps/sssp/kernel.cu:472
     s.delta = delta;
     s.aux[1] = a.far_mode;
+    s.aux[2] = a.expand_mode;
     c->s = s;
-    c->stat_items = c->stat_rounds = c->stat_heavy = c->stat_phases = 0;
+    c->stat_items = c->stat_rounds = c->stat_heavy = c->stat_phases = c->stat_edges = 0;
   }
 }
 
@@ -463,6 +525,7 @@ static int check_single_opts(const lsg_opts &o, bool sssp, int *bs, int *expand)
   if (*expand != LSG_EXPAND_TWB && *expand != LSG_EXPAND_LB) return fail(LSG_ERR_ARG, "bad expand_mode %d", *expand);
   if (o.far_mode != LSG_FAR_REFERENCE && o.far_mode != LSG_FAR_SPLIT)
     return fail(LSG_ERR_ARG, "far_mode %d is not available for single-source runs (reference or split)", o.far_mode);
+  if (o.sched && !sssp) return fail(LSG_ERR_ARG, "the learned scheduler chooses SSSP delta widths; BFS has none");
   return LSG_OK;
 }
 
@@ -509,11 +572,20 @@ static int run_core(Workspace *w, const CsrView &g, const uint32_t *seeds, uint3
   a.far_mode = (uint32_t)o.far_mode;
   a.spin_limit = spin_limit_ticks(w);
   a.residency_limit = residency_limit_ticks(w);
+  a.nedges = g.nedges;
+  if (o.sched) {
+    a.sched = 1;
+    a.mlp = sched_model_device(w->device, LSG_SCHED_PLAIN);
+    if (!a.mlp)
+      return fail(LSG_ERR_ARG, "learned scheduler requested but no plain-kernel model is loaded (lsg_sched_set_model)");
+  }
   if (o.trace) {
     void *tbuf;
-    if ((rc = ws_buffer(w, kBufTrace, kTraceCap * sizeof(lsg_round_trace), &tbuf))) return rc;
+    const size_t tbytes = kTraceCap * sizeof(lsg_round_trace);
+    if ((rc = ws_buffer(w, kBufTrace, tbytes + kMaxForced * kFeatRec * sizeof(float), &tbuf))) return rc;
     a.trace = (lsg_round_trace *)tbuf;
     a.trace_cap = kTraceCap;
+    if (A == Algo::SSSP) a.feat = (float *)((char *)tbuf + tbytes);
   }
   HIPCHK(hipMemcpyAsync(seedbuf, seeds, (size_t)nseeds * 4, hipMemcpyHostToDevice, w->stream));
 
@@ -569,6 +641,10 @@ static int run_core(Workspace *w, const CsrView &g, const uint32_t *seeds, uint3
     uint32_t n = (uint32_t)std::min<uint64_t>(hc.stat_rounds, kTraceCap);
     w->last_trace.resize(n);
     if (n) HIPCHK(hipMemcpy(w->last_trace.data(), a.trace, n * sizeof(lsg_round_trace), hipMemcpyDeviceToHost));
+    // one feature record per phase boundary: phase p -> p + 1 is record p
+    uint32_t np = a.feat ? std::min<uint32_t>(hc.s.phase, kMaxForced) : 0u;
+    w->last_feat.resize((size_t)np * kFeatRec);
+    if (np) HIPCHK(hipMemcpy(w->last_feat.data(), a.feat, w->last_feat.size() * sizeof(float), hipMemcpyDeviceToHost));
   }
   if (st) {
     memset(st, 0, sizeof *st);

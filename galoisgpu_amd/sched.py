@@ -1,15 +1,22 @@
 """Learned phase scheduler: model files, forced schedules, trace sweeps and training.
 
-The model (layout in csrc/sched_mlp.h) maps 16 frontier features, taken by the
+A model (layout in csrc/sched_mlp.h) maps 16 frontier features, taken by a
 persistent kernel's barrier leader at every phase boundary, to one of 8
-actions: delta multiplier in {1/4, 1, 8, 2^24 (unbounded)} x far-pile mode in
-{reference (one shared threshold), bucket (per-source thresholds)}.  Training data comes
-from the plain method itself: for each training
-instance the kernel is run under a sweep of forced per-phase schedules, the
-fastest one is kept, and the features it recorded at each phase are labelled
-with the action it took there (``sweep_instance``).  ``train`` fits the MLP with
-the HIP training kernel (lsg_mlp_train).  Whatever the model picks, distances
-are identical: the actions only reorder the same u32 relaxations.
+actions: a delta multiplier in {1/4, 1, 8, 2^24 (unbounded)} times a mode.
+There is one model per kernel kind:
+
+* ``"plain"`` -- the single-source kernel (the drop-in for apps/sssp); the mode
+  is the expansion mode, degree-binned (TWB) or tile-flattened (LB), the
+  reference's np_schedulers / -l choice (apps/sssp/kernel.cu:10,434-453).
+* ``"batched"`` -- the multi-source kernel; the mode is the far-pile mode,
+  reference (one shared threshold) or bucket (per-source thresholds).
+
+Training data comes from the kernel itself: for each training instance it is
+run under a sweep of forced per-phase schedules, the fastest one is kept, and
+the features it recorded at each phase are labelled with the action it took
+there (``sweep_instance``).  ``train`` fits the MLP with the HIP training
+kernel (lsg_mlp_train).  Whatever the model picks, distances are identical:
+the actions only reorder the same u32 relaxations.
 """
 from __future__ import annotations
 
@@ -21,16 +28,21 @@ from typing import Dict, List, Optional, Sequence, Tuple
 
 import numpy as np
 
-from ._lib import check, lib
+from ._lib import EXPAND_LB, EXPAND_TWB, SCHED_BATCHED, SCHED_PLAIN, check, lib
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 N_FEAT, N_HID, N_ACT, MAX_FORCED = 16, 64, 8, 256
 DELTA_MULTS = [0.25, 1.0, 8.0, float(1 << 24)]  # the last one makes a phase plain Bellman-Ford rounds
 FAR_REFERENCE, FAR_SPLIT, FAR_BUCKET = 0, 1, 2
-# action id = 4 * (far mode is bucket) + delta multiplier index
-ACTIONS = [(m, far) for far in (FAR_REFERENCE, FAR_BUCKET) for m in DELTA_MULTS]
-FIXED_ACTION = 1  # delta x1, reference far pile: the plain method's schedule
+KINDS = {"batched": SCHED_BATCHED, "plain": SCHED_PLAIN}
+# action id = 4 * (mode index) + delta multiplier index
+ACTIONS_BY_KIND = {
+    "batched": [(m, far) for far in (FAR_REFERENCE, FAR_BUCKET) for m in DELTA_MULTS],
+    "plain": [(m, ex) for ex in (EXPAND_TWB, EXPAND_LB) for m in DELTA_MULTS],
+}
+ACTIONS = ACTIONS_BY_KIND["batched"]
+FIXED_ACTION = 1  # delta x1 with the first mode: the plain method's schedule
 
 OFF_MEAN = 0
 OFF_INVSTD = OFF_MEAN + N_FEAT
@@ -52,11 +64,11 @@ def model_floats() -> int:
     return n
 
 
-def default_model_path() -> str:
-    return os.path.join(HERE, "models", "sched_model.npz")
+def default_model_path(kind: str = "batched") -> str:
+    return os.path.join(HERE, "models", f"sched_{kind}.npz")
 
 
-def new_model(seed: int = 0) -> np.ndarray:
+def new_model(seed: int = 0, kind: str = "batched") -> np.ndarray:
     """Randomly initialised network (He init), action tables, no forced schedule."""
     rng = np.random.default_rng(seed)
     m = np.zeros(MODEL_FLOATS, dtype=np.float32)
@@ -64,15 +76,16 @@ def new_model(seed: int = 0) -> np.ndarray:
     m[OFF_W1:OFF_B1] = rng.normal(0, np.sqrt(2.0 / N_FEAT), N_FEAT * N_HID)
     m[OFF_W2:OFF_B2] = rng.normal(0, np.sqrt(2.0 / N_HID), N_HID * N_HID)
     m[OFF_W3:OFF_B3] = rng.normal(0, np.sqrt(1.0 / N_HID), N_HID * N_ACT)
-    for a, (mult, far) in enumerate(ACTIONS):
+    for a, (mult, mode) in enumerate(ACTIONS_BY_KIND[kind]):
         m[OFF_DELTA + a] = mult
-        m[OFF_FAR + a] = far
+        m[OFF_FAR + a] = mode
     m[OFF_FORCED:OFF_FORCED + MAX_FORCED] = -1.0
     return m
 
 
 def forced(model: np.ndarray, actions: Sequence[int]) -> np.ndarray:
-    """Copy of `model` whose phase p takes actions[min(p, len-1)] regardless of the MLP."""
+    """Copy of `model` whose phase p takes actions[min(p, len-1)] regardless of
+    the MLP (the kernel holds the last of the MAX_FORCED entries for later phases)."""
     m = model.copy()
     acts = list(actions) or [FIXED_ACTION]
     for p in range(MAX_FORCED):
@@ -80,9 +93,10 @@ def forced(model: np.ndarray, actions: Sequence[int]) -> np.ndarray:
     return m
 
 
-def set_model(model: np.ndarray, device: int = 0) -> None:
+def set_model(model: np.ndarray, device: int = 0, kind: str = "batched") -> None:
     m = np.ascontiguousarray(model, dtype=np.float32)
-    check(lib().lsg_sched_set_model(device, m.ctypes.data_as(ctypes.c_void_p), len(m)), "sched_set_model")
+    check(lib().lsg_sched_set_model(device, KINDS[kind], m.ctypes.data_as(ctypes.c_void_p), len(m)),
+          "sched_set_model")
 
 
 def save_model(path: str, model: np.ndarray, meta: dict) -> None:
@@ -90,14 +104,19 @@ def save_model(path: str, model: np.ndarray, meta: dict) -> None:
     np.savez(path, model=np.asarray(model, dtype=np.float32), meta=np.array(json.dumps(meta)))
 
 
-def load_model(path: Optional[str] = None, device: int = 0) -> Tuple[np.ndarray, dict]:
-    z = np.load(path or default_model_path(), allow_pickle=False)
+def load_model(path: Optional[str] = None, device: int = 0, kind: Optional[str] = None
+               ) -> Tuple[np.ndarray, dict]:
+    """Loads a saved model onto `device` for its kind (from the file's metadata
+    unless `kind` is given)."""
+    z = np.load(path or default_model_path(kind or "batched"), allow_pickle=False)
     model = z["model"].astype(np.float32)
+    meta = json.loads(str(z["meta"]))
     if len(model) != MODEL_FLOATS:
         raise ValueError(f"{path}: model has {len(model)} floats, expected {MODEL_FLOATS}")
+    kind = kind or meta.get("kind", "batched")
     model[OFF_FORCED:OFF_FORCED + MAX_FORCED] = -1.0  # a stored model never forces phases
-    set_model(model, device)
-    return model, json.loads(str(z["meta"]))
+    set_model(model, device, kind)
+    return model, meta
 
 
 def infer(model: np.ndarray, X: np.ndarray, device: int = 0) -> np.ndarray:
@@ -128,18 +147,24 @@ def train(X: np.ndarray, y: np.ndarray, w: Optional[np.ndarray] = None, steps: i
 
 
 def run_schedule(dev, srcs, base_delta: int, schedule: Sequence[int], model: np.ndarray, device: int = 0,
-                 repeats: int = 2, **opt) -> Tuple[float, np.ndarray]:
-    """Runs the batched kernel under a forced per-phase schedule.  Returns the
-    best kernel time (ms) over `repeats` runs and the per-phase feature records."""
-    from .apps import Options
+                 repeats: int = 2, kind: str = "batched", **opt) -> Tuple[float, np.ndarray]:
+    """Runs the kernel of `kind` under a forced per-phase schedule (the plain
+    kernel from srcs[0]).  Returns the best kernel time (ms) over `repeats`
+    runs and the per-phase feature records."""
+    from .apps import Options, sssp_into
     from .batch import DeviceArray, last_features, sssp_multi_into
-    set_model(forced(model, schedule), device)
-    D = DeviceArray(dev.nnodes * len(srcs) * 4, device)
+    set_model(forced(model, schedule), device, kind)
+    nsrc = len(srcs) if kind == "batched" else 1
+    D = DeviceArray(dev.nnodes * nsrc * 4, device)
     try:
         best = float("inf")
         feats = None
         for _ in range(repeats):
-            st = sssp_multi_into(dev, srcs, Options(delta=base_delta, sched=1, trace=True, **opt), D)
+            if kind == "batched":
+                st = sssp_multi_into(dev, srcs, Options(delta=base_delta, sched=1, trace=True, **opt), D)
+            else:
+                st = sssp_into(dev, D.ptr, Options(delta=base_delta, start_node=int(srcs[0]), sched=1, trace=True,
+                                                   **opt))
             if st["ms_kernel"] < best:
                 best = st["ms_kernel"]
                 feats = last_features(device)
@@ -149,7 +174,7 @@ def run_schedule(dev, srcs, base_delta: int, schedule: Sequence[int], model: np.
 
 
 def sweep_instance(dev, srcs, base_delta: int, model: np.ndarray, max_greedy_phases: int = 12, device: int = 0,
-                   log=None, **opt) -> Tuple[np.ndarray, np.ndarray, dict]:
+                   log=None, kind: str = "batched", **opt) -> Tuple[np.ndarray, np.ndarray, dict]:
     """Labels one training instance by sweeping schedules of the plain method.
 
     Every constant action is timed; then, phase by phase (up to
@@ -163,7 +188,7 @@ def sweep_instance(dev, srcs, base_delta: int, model: np.ndarray, max_greedy_pha
     def timed(sch):
         key = tuple(sch)
         if key not in times:
-            times[key] = run_schedule(dev, srcs, base_delta, sch, model, device, **opt)
+            times[key] = run_schedule(dev, srcs, base_delta, sch, model, device, kind=kind, **opt)
         return times[key]
 
     fixed_ms, _ = timed([FIXED_ACTION])
@@ -191,8 +216,18 @@ def sweep_instance(dev, srcs, base_delta: int, model: np.ndarray, max_greedy_pha
     return X, y, summary
 
 
-def training_instances(quick: bool = False) -> List[dict]:
+def training_instances(quick: bool = False, kind: str = "batched") -> List[dict]:
     """Graphs the scheduler is trained on (none of them is a benchmark graph)."""
+    if kind == "plain":
+        inst = [dict(kind="road", rows=384, cols=384, seed=21, nsrc=1, delta=10000),
+                dict(kind="road", rows=512, cols=512, seed=22, nsrc=1, delta=2000),
+                dict(kind="road", rows=768, cols=768, seed=23, nsrc=1, delta=10000),
+                dict(kind="grid", rows=512, cols=512, seed=24, nsrc=1, delta=32),
+                dict(kind="grid", rows=256, cols=1024, seed=25, nsrc=1, delta=100),
+                dict(kind="rmat", scale=16, seed=26, nsrc=1, delta=32),
+                dict(kind="rmat", scale=18, seed=27, nsrc=1, delta=32),
+                dict(kind="rmat", scale=17, seed=28, nsrc=1, delta=512)]
+        return inst[:3] if quick else inst
     inst = []
     for scale, seed in ((14, 11), (16, 12), (17, 13), (18, 14)):
         inst.append(dict(kind="rmat", scale=scale, seed=seed, nsrc=64, delta=32))
@@ -218,16 +253,17 @@ def build_instance(spec: dict):
     return g, srcs
 
 
-def train_from_sweeps(out_path: str, quick: bool = False, device: int = 0, steps: int = 6000, log=print) -> dict:
+def train_from_sweeps(out_path: str, quick: bool = False, device: int = 0, steps: int = 6000, log=print,
+                      kind: str = "batched") -> dict:
     """Full pipeline: sweep every training instance, train the MLP with the
     HIP trainer, save the model with its provenance."""
-    base = new_model(0)
+    base = new_model(0, kind)
     Xs, ys, ws, summaries = [], [], [], []
-    for spec in training_instances(quick):
+    for spec in training_instances(quick, kind):
         g, srcs = build_instance(spec)
         dev = g.copy_to_gpu(device)
         t0 = time.time()
-        X, y, summ = sweep_instance(dev, srcs, spec["delta"], base, device=device, log=log)
+        X, y, summ = sweep_instance(dev, srcs, spec["delta"], base, device=device, log=log, kind=kind)
         dev.close()
         gain = summ["fixed_ms"] / max(summ["best_ms"], 1e-6)
         summ.update(spec=spec, sweep_s=round(time.time() - t0, 1), gain=round(gain, 3))
@@ -240,10 +276,10 @@ def train_from_sweeps(out_path: str, quick: bool = False, device: int = 0, steps
     X = np.concatenate(Xs)
     y = np.concatenate(ys)
     w = np.concatenate(ws)
-    model, loss = train(X, y, w, steps=steps, device=device)
+    model, loss = train(X, y, w, steps=steps, device=device, model=base)
     acc = float((infer(model, X, device) == y).mean())
-    meta = {"instances": summaries, "samples": int(len(y)), "train_loss": loss, "train_acc": acc,
-            "actions": ACTIONS, "steps": steps, "created": time.strftime("%Y-%m-%d %H:%M:%S")}
+    meta = {"kind": kind, "instances": summaries, "samples": int(len(y)), "train_loss": loss, "train_acc": acc,
+            "actions": ACTIONS_BY_KIND[kind], "steps": steps, "created": time.strftime("%Y-%m-%d %H:%M:%S")}
     save_model(out_path, model, meta)
     log(f"trained on {len(y)} phase samples: loss {loss:.4f}, accuracy {acc:.3f} -> {out_path}")
     return meta
@@ -264,8 +300,8 @@ def profiled_traffic(root: str, name: str) -> Optional[float]:
     return None
 
 
-__all__ = ["ACTIONS", "FIXED_ACTION", "MODEL_FLOATS", "new_model", "forced", "set_model", "save_model",
-           "load_model", "infer", "train", "default_model_path", "profiled_traffic"]
+__all__ = ["ACTIONS", "ACTIONS_BY_KIND", "KINDS", "FIXED_ACTION", "MODEL_FLOATS", "new_model", "forced", "set_model",
+           "save_model", "load_model", "infer", "train", "default_model_path", "profiled_traffic"]
 
 
 if __name__ == "__main__":
@@ -273,8 +309,10 @@ if __name__ == "__main__":
 
     ap = argparse.ArgumentParser(description="train the learned phase scheduler on the GPU")
     ap.add_argument("command", choices=["train"])
-    ap.add_argument("--out", default=default_model_path())
+    ap.add_argument("--kind", choices=sorted(KINDS), default="batched")
+    ap.add_argument("--out", default=None)
     ap.add_argument("--quick", action="store_true")
     ap.add_argument("--steps", type=int, default=6000)
     a = ap.parse_args()
-    train_from_sweeps(a.out, quick=a.quick, steps=a.steps, log=lambda *x: print(*x, flush=True))
+    train_from_sweeps(a.out or default_model_path(a.kind), quick=a.quick, steps=a.steps, kind=a.kind,
+                      log=lambda *x: print(*x, flush=True))

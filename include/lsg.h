@@ -26,7 +26,7 @@ extern "C" {
 #endif
 
 #define LSG_INF 0x7FFFFFFF /* INT_MAX, apps/sssp/kernel.cu:18 */
-#define LSG_ABI_VERSION 1
+#define LSG_ABI_VERSION 2
 
 enum {
   LSG_OK = 0,
@@ -85,14 +85,13 @@ typedef struct lsg_opts {
   int32_t grid_blocks; /* persistent grid; 0 = every block the device can keep resident */
   int32_t thread_deg;  /* TWB: degrees below this are expanded by one lane (default 8) */
   int32_t heavy_deg;   /* degrees >= this are split into chunks of heavy_deg edges (default 4096) */
-  int32_t sched;       /* 0 = fixed delta, 1 = learned per-phase delta (lsg_sched_set_model) */
+  int32_t sched;       /* 0 = fixed delta; 1 = the learned scheduler picks each phase's delta width
+                          and mode (expansion mode for SSSP, far mode for batched runs) from the
+                          model loaded with lsg_sched_set_model */
   int32_t trace;       /* 1 = record one lsg_round_trace per round (lsg_last_trace) */
   int32_t graph_launch; /* 1 = launch the kernel sequence as a captured hipGraph (replayed while
                            the arguments stay the same) */
-  int32_t pull_permille; /* batched: a round whose frontier holds at least this many per mille of
-                            the vertices gathers along in-edges (pull) instead of pushing along
-                            out-edges; 0 = always push */
-  int32_t reserved[3];
+  int32_t reserved[4];
 } lsg_opts;
 
 /* One record per round of a single-source traversal, written when
@@ -220,16 +219,23 @@ int lsg_sssp_batch(const lsg_graph *g, const uint32_t *srcs, uint32_t nsrc, cons
  * single-source distances.  dist_out[i] is a device array of graphs[i].nnodes. */
 int lsg_sssp_graphs(const lsg_graph *graphs, uint32_t ngraphs, const uint32_t *srcs, const lsg_opts *opts,
                     int32_t *const *dist_out, lsg_batch_stats *stats);
-/* Per-phase feature records of the last traced batched call: nphases rows of
- * 20 floats (16 scheduler features, action or -1, delta, far mode, first round). */
+/* Per-phase feature records of the last traced SSSP call (single-source or
+ * batched): nphases rows of 20 floats (16 scheduler features, action or -1,
+ * delta, mode -- expansion mode single-source, far mode batched -- and the
+ * first round of the phase). */
 int lsg_last_features(int device, float *out, uint32_t cap_phases, uint32_t *nphases);
 
 /* ---- learned scheduler (no reference counterpart) ------------------------- */
 /* Size in floats of a scheduler model (normalisation, 16-64-64-8 MLP weights,
  * per-action delta multiplier and far mode, forced per-phase actions). */
 int lsg_sched_model_floats(void);
-/* Loads the model used when lsg_opts.sched = 1 on `device`. */
-int lsg_sched_set_model(int device, const float *model, uint32_t nfloats);
+/* Model kinds: one model per kind and device. */
+enum {
+  LSG_SCHED_BATCHED = 0, /* lsg_sssp_multi / _batch / _graphs: actions pick delta x far mode */
+  LSG_SCHED_PLAIN = 1    /* single-source SSSP: actions pick delta x expansion mode */
+};
+/* Loads the model of `kind` used when lsg_opts.sched = 1 on `device`. */
+int lsg_sched_set_model(int device, int kind, const float *model, uint32_t nfloats);
 /* Trains the MLP part of `model` in place on n samples (X: n x 16 raw
  * features, y: action labels, weights: per-sample loss weights or NULL) with
  * minibatch SGD + momentum in a HIP kernel; also sets the feature

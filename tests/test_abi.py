@@ -40,7 +40,7 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(_lib.LsgGraph) == 4 + 4 + 4 * 8 + 4 + 4
     assert ctypes.sizeof(_lib.LsgOpts) == 16 * 4
     assert ctypes.sizeof(_lib.LsgStats) == 8 * 6 + 4 * 8
-    assert _lib.lib().lsg_abi_version() == 1
+    assert _lib.lib().lsg_abi_version() == 2
 
 
 def test_defaults_mirror_support_cu():

@@ -122,6 +122,38 @@ def test_roadlike_config3():
     dev.close()
 
 
+def test_plain_learned_schedules_keep_distances():
+    """The single-source kernel under forced and learned per-phase schedules
+    (delta width x expansion mode) gives exactly the oracle's distances."""
+    from galoisgpu_amd import sched as lsched
+    cases = [(CSRGraph.roadlike(192, 192, 0.10, 0.001, 1, 10000, seed=5), 2000),
+             (CSRGraph.rmat(14, 16, seed=5), 32)]
+    base = lsched.new_model(0, kind="plain")
+    rng = np.random.default_rng(3)
+    schedules = [[a] for a in range(lsched.N_ACT)] + [list(rng.integers(0, lsched.N_ACT, 300)) for _ in range(2)]
+    for g, delta in cases:
+        src = g.max_degree_vertex()
+        ref = oracle.dijkstra(g.row_start, g.edge_dst, g.edge_data, src)
+        dev = g.copy_to_gpu()
+        for sch in schedules:
+            lsched.set_model(lsched.forced(base, sch), kind="plain")
+            say("plain forced", g.nnodes, sch[:6])
+            d = run_sssp(g, dev, src, delta, sched=1, trace=True)
+            assert np.array_equal(d, ref), sch
+            from galoisgpu_amd.batch import last_features
+            feats = last_features()
+            assert len(feats) > 0
+            acts = feats[:, 16].astype(int)
+            assert all(acts[p] == sch[min(p, len(sch) - 1)] for p in range(len(acts)))
+            modes = feats[:, 18].astype(int)
+            assert all(modes[p] == lsched.ACTIONS_BY_KIND["plain"][acts[p]][1] for p in range(len(acts)))
+        lsched.set_model(base, kind="plain")  # an untrained network as the scheduler: still exact
+        assert np.array_equal(run_sssp(g, dev, src, delta, sched=1), ref)
+        with pytest.raises(LsgError, match="BFS"):
+            run_bfs(g, dev, src, sched=1)
+        dev.close()
+
+
 def test_rmat22_sources_certificate():
     g = CSRGraph.rmat(22, 16, seed=1)
     dev = g.copy_to_gpu()

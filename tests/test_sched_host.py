@@ -25,6 +25,27 @@ def test_new_model_action_tables():
     assert np.isfinite(m).all()
 
 
+def test_plain_model_action_tables():
+    m = lsched.new_model(3, kind="plain")
+    for a, (mult, mode) in enumerate(lsched.ACTIONS_BY_KIND["plain"]):
+        assert m[lsched.OFF_DELTA + a] == np.float32(mult)
+        assert m[lsched.OFF_FAR + a] == mode  # expansion mode: 0 TWB, 1 LB
+    assert lsched.ACTIONS_BY_KIND["plain"][lsched.FIXED_ACTION] == (1.0, 0)  # delta x1, degree-binned
+    assert sorted({mode for _, mode in lsched.ACTIONS_BY_KIND["plain"]}) == [0, 1]
+
+
+def test_set_model_rejects_bad_kind_and_size():
+    from galoisgpu_amd._lib import LSG_OK, lib
+    import ctypes
+    m = np.zeros(lsched.MODEL_FLOATS, dtype=np.float32)
+    p = m.ctypes.data_as(ctypes.c_void_p)
+    assert lib().lsg_sched_set_model(0, 7, p, len(m)) != LSG_OK
+    assert lib().lsg_sched_set_model(0, 1, p, len(m) - 1) != LSG_OK
+    bad = m.copy()
+    bad[5] = np.nan
+    assert lib().lsg_sched_set_model(0, 1, bad.ctypes.data_as(ctypes.c_void_p), len(m)) != LSG_OK
+
+
 def test_forced_schedule_extends_last_action():
     m = lsched.forced(lsched.new_model(0), [5, 2, 7])
     f = m[lsched.OFF_FORCED:lsched.OFF_FORCED + lsched.MAX_FORCED]
