@@ -287,8 +287,6 @@ __device__ __forceinline__ void process_chunk(const MArgs &a, const Sched &s, La
   }
 }
 
-__device__ __forceinline__ void stat_add64(uint64_t *p, uint64_t v) { st_coh64(p, ld_coh64(p) + v); }
-
 // Leader step, run by every thread of the last block to reach the barrier:
 // the pile rotation of apps/sssp/kernel.cu:363-384 for all sources at once,
 // plus, at each phase boundary, the learned scheduler's choice of the next
@@ -299,24 +297,28 @@ __device__ void multi_leader(const MArgs &a, MultiCounters *tot) {
   __shared__ SchedFeatures f;
   __shared__ MlpScratch sc;
   __shared__ uint32_t phase_end, consumed;
+  __shared__ uint64_t phases_seen;
   Ctrl *c = a.ctrl;
   const uint32_t t = threadIdx.x;
   if (t == 0) {
+    // every control word of the step in one batch of loads
     sched_load(s, &c->s);
+    CtrlCounts k;
+    counts_load(k, c);
+    const uint64_t tot_lane_items = ld_coh64(&tot->lane_items), tot_edges = ld_coh64(&tot->edges);
     // every wavefront has left the round's item loops: restart the tickets
     st_coh(&c->ticket, 0u);
     st_coh(&c->ticket2, 0u);
-    uint32_t nnear = ld_coh(&c->qcount[s.near_q]);
-    uint32_t nfar = ld_coh(&c->qcount[s.far_q]);
-    uint32_t nheavy = ld_coh(&c->hcount[1u - s.heavy_in]);
+    const uint32_t nnear = k.q(s.near_q), nfar = k.q(s.far_q), nheavy = k.h(1u - s.heavy_in);
     phase_end = 0;
+    phases_seen = k.phases;
     if (nnear > a.qcap || nfar > a.qcap || nheavy > a.hcap) {
       st_coh(&c->abort, kAbortCapacity);
       s.done = 1;
     } else {
-      stat_add64(&c->stat_items, s.in_count);
-      stat_add64(&c->stat_heavy, s.heavy_count);
-      stat_add64(&c->stat_rounds, 1);
+      st_coh64(&c->stat_items, k.items + s.in_count);
+      st_coh64(&c->stat_heavy, k.heavy + s.heavy_count);
+      st_coh64(&c->stat_rounds, k.rounds + 1);
       if (a.trace && s.round - 1 < a.trace_cap) {
         lsg_round_trace &r = a.trace[s.round - 1];
         r.round = s.round;
@@ -350,14 +352,14 @@ __device__ void multi_leader(const MArgs &a, MultiCounters *tot) {
         f.curdelta = (float)s.curdelta();
         f.base_delta = a.delta;
         f.prev_delta = s.delta;
-        f.lane_items = (float)ld_coh64(&tot->lane_items);
-        f.edges = (float)ld_coh64(&tot->edges);
+        f.lane_items = (float)tot_lane_items;
+        f.edges = (float)tot_edges;
         f.nnodes = a.nnodes;
         f.nedges = (float)a.nedges;
         f.nsrc = a.S;
       } else {
         s.done = 1;
-        stat_add64(&c->stat_phases, 1);
+        st_coh64(&c->stat_phases, k.phases + 1);
       }
     }
   }
@@ -404,7 +406,7 @@ __device__ void multi_leader(const MArgs &a, MultiCounters *tot) {
     s.set_curdelta(s.curdelta() + s.delta);
     s.phase += 1;
     s.round += 1;
-    stat_add64(&c->stat_phases, 1);
+    st_coh64(&c->stat_phases, phases_seen + 1);
     sched_store(&c->s, s);
   }
 }
@@ -501,10 +503,9 @@ __global__ void multi_seed(MArgs a) {
   }
   uint64_t pm = __ballot(push);
   if (push) a.q[0][lanes_below(pm)] = src;
+  ctrl_reset_barrier(a.ctrl, lane, kWave);
   if (lane == 0) {
     Ctrl *c = a.ctrl;
-    c->bar_count = 0;
-    c->bar_gen = 0;
     c->qcount[0] = (uint32_t)__builtin_popcountll(pm);
     c->qcount[1] = 0;
     c->qcount[2] = 0;
@@ -531,7 +532,7 @@ __global__ void multi_seed(MArgs a) {
       a.fmin[64 + k] = kInf;
       a.thr[k] = 0u;  // phase 0: threshold 0, as curdelta starts at 0 (apps/sssp/kernel.cu:472)
     }
-    c->stat_items = c->stat_rounds = c->stat_heavy = c->stat_phases = 0;
+    c->stat_items = c->stat_rounds = c->stat_heavy = c->stat_phases = c->stat_edges = 0;
   }
 }
 
@@ -648,7 +649,7 @@ static int run_multi(Workspace *w, const lsg_graph *g, const uint32_t *srcs_host
       return rc;
     }
     HIPCHK(hipEventRecord(w->ev1, w->stream));
-    HIPCHK(hipMemcpyAsync(&hc, w->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, w->stream));
+    HIPCHK(hipMemcpyAsync(&hc, w->ctrl, kCtrlHostBytes, hipMemcpyDeviceToHost, w->stream));
     HIPCHK(hipStreamSynchronize(w->stream));
     HIPCHK(hipEventElapsedTime(&ms, w->ev0, w->ev1));
     if (hc.abort != kAbortResidency) break;

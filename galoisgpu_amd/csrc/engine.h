@@ -16,6 +16,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 
 namespace lsg {
@@ -106,7 +107,16 @@ __device__ __forceinline__ void sched_store(Sched *dst, const Sched &s) {
   for (int i = 0; i < kSchedWords; ++i) st_coh(p + i, d[i]);
 }
 
-// Device control block.  Each hot word sits on its own 128-byte line.
+// Grid barrier arrival tree: blocks arrive in groups of kBarGroup on a
+// group counter; the last of each group arrives on the top counter.  One
+// counter for every block costs ~25 us per barrier at 2048 blocks on gfx950
+// (same-address atomics serialise), the two-level tree ~3 us.
+constexpr uint32_t kBarGroup = 32;
+constexpr uint32_t kMaxBarGroups = 128;
+constexpr uint32_t kMaxGridBlocks = kBarGroup * kMaxBarGroups;
+
+// Device control block.  Each hot word sits on its own 128-byte line.  The
+// host copies back the part before bar_grp (kCtrlHostBytes) after a launch.
 struct Ctrl {
   uint32_t bar_count; uint32_t _p0[31];
   uint32_t bar_gen;   uint32_t _p1[31];
@@ -117,7 +127,40 @@ struct Ctrl {
   Sched s;            uint32_t _p6[16];
   uint64_t stat_items, stat_rounds, stat_heavy, stat_phases, stat_edges;
   uint32_t _p7[22];
+  uint32_t bar_grp[kMaxBarGroups][32];  // group arrival counters, one line each
 };
+constexpr size_t kCtrlHostBytes = offsetof(Ctrl, bar_grp);
+
+// Zeroes the barrier counters; run by the init kernels before each launch
+// (tid / nthreads: the caller's grid-stride position).
+__device__ __forceinline__ void ctrl_reset_barrier(Ctrl *c, size_t tid, size_t nthreads) {
+  for (size_t g = tid; g < kMaxBarGroups; g += nthreads) c->bar_grp[g][0] = 0;
+  if (tid == 0) {
+    c->bar_count = 0;
+    c->bar_gen = 0;
+  }
+}
+
+// Queue counters and statistics words of the control block, loaded together
+// by a barrier leader.
+struct CtrlCounts {
+  uint32_t qc[3], hc[2];
+  uint64_t items, rounds, heavy, phases, edges;
+  __device__ uint32_t q(uint32_t i) const { return i == 0 ? qc[0] : i == 1 ? qc[1] : qc[2]; }
+  __device__ uint32_t h(uint32_t i) const { return i == 0 ? hc[0] : hc[1]; }
+};
+__device__ __forceinline__ void counts_load(CtrlCounts &k, const Ctrl *c) {
+  k.qc[0] = ld_coh(&c->qcount[0]);
+  k.qc[1] = ld_coh(&c->qcount[1]);
+  k.qc[2] = ld_coh(&c->qcount[2]);
+  k.hc[0] = ld_coh(&c->hcount[0]);
+  k.hc[1] = ld_coh(&c->hcount[1]);
+  k.items = ld_coh64(&c->stat_items);
+  k.rounds = ld_coh64(&c->stat_rounds);
+  k.heavy = ld_coh64(&c->stat_heavy);
+  k.phases = ld_coh64(&c->stat_phases);
+  k.edges = ld_coh64(&c->stat_edges);
+}
 
 // ---- wave64 helpers --------------------------------------------------------
 
@@ -171,9 +214,22 @@ __device__ __forceinline__ uint32_t wave_reserve(uint32_t n, uint32_t *tail) {
 
 // ---- grid barrier ----------------------------------------------------------
 
-// Leader grid barrier.  Every block arrives on one counter; the last to arrive
-// runs `leader()` (single thread), resets the counter and bumps the
-// generation; the others spin on the generation.  Replaces GlobalBarrier::Sync
+// One block's arrival (thread 0): true in the block that completes the barrier.
+// The counters only ever grow: a group's barrier completes when its count
+// reaches a multiple of the group size, the top level when its count reaches
+// a multiple of the number of groups.  Arrivals of the next barrier cannot
+// start before this one's release, so no reset is needed.
+__device__ __forceinline__ bool barrier_arrive(Ctrl *c, uint32_t nblocks) {
+  const uint32_t g = blockIdx.x / kBarGroup;
+  const uint32_t gsize = min(kBarGroup, nblocks - g * kBarGroup);
+  const uint32_t ngroups = (nblocks + kBarGroup - 1) / kBarGroup;
+  if ((atomic_add(&c->bar_grp[g][0], 1u) + 1) % gsize != 0) return false;
+  return (atomic_add(&c->bar_count, 1u) + 1) % ngroups == 0;
+}
+
+// Leader grid barrier.  Blocks arrive through barrier_arrive; the last to
+// arrive runs `leader()` (single thread) and bumps the generation; the others
+// spin on the generation.  Replaces GlobalBarrier::Sync
 // (rt/include/gbar.cuh:75-121).  The spin is bounded by `spin_limit` ticks of
 // the constant-rate wall clock; when exceeded, or when another block has
 // aborted, the block raises kAbortTimeout and returns so every wave exits.
@@ -183,11 +239,9 @@ __device__ __forceinline__ void grid_barrier(Ctrl *c, uint32_t nblocks, uint64_t
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t gen = ld_coh(&c->bar_gen);
-    uint32_t arrived = atomic_add(&c->bar_count, 1u);
-    if (arrived == nblocks - 1) {
+    if (barrier_arrive(c, nblocks)) {
       order_acquire();
       leader();
-      st_coh(&c->bar_count, 0u);
       order_release();
       st_coh(&c->bar_gen, gen + 1);
     } else {
@@ -216,8 +270,7 @@ __device__ __forceinline__ void grid_barrier_block(Ctrl *c, uint32_t nblocks, ui
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t gen = ld_coh(&c->bar_gen);
-    uint32_t arrived = atomic_add(&c->bar_count, 1u);
-    s_lead = arrived == nblocks - 1;
+    s_lead = barrier_arrive(c, nblocks);
     s_gen = gen;
   }
   __syncthreads();
@@ -226,11 +279,7 @@ __device__ __forceinline__ void grid_barrier_block(Ctrl *c, uint32_t nblocks, ui
     leader();
     order_release();
     __syncthreads();
-    if (threadIdx.x == 0) {
-      st_coh(&c->bar_count, 0u);
-      order_release();
-      st_coh(&c->bar_gen, s_gen + 1);
-    }
+    if (threadIdx.x == 0) st_coh(&c->bar_gen, s_gen + 1);
   } else if (threadIdx.x == 0) {
     uint32_t gen = s_gen;
     uint64_t t0 = wall_clock64();

@@ -71,7 +71,7 @@ int resident_grid(Workspace *w, const void *fn, int bs, int *grid) {
   int per_cu = 0;
   HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, bs, 0));
   if (per_cu <= 0) return fail(LSG_ERR_HIP, "persistent kernel cannot be resident (block %d)", bs);
-  *grid = per_cu * w->cus;
+  *grid = std::min<int>(per_cu * w->cus, (int)kMaxGridBlocks);  // the barrier tree's capacity
   w->grids.push_back({fn, bs, *grid});
   return LSG_OK;
 }

@@ -292,8 +292,6 @@ __device__ void process_heavy(const KArgs &a, const Sched &s, uint32_t c, TileSh
   }
 }
 
-__device__ __forceinline__ void stat_add(uint64_t *p, uint64_t v) { st_coh64(p, ld_coh64(p) + v); }
-
 // Leader step between rounds, run by every thread of the last block to reach
 // the barrier.  Implements the pile rotation of the persistent driver
 // (apps/sssp/kernel.cu:363-384): iterate the near pile to empty (retry2), then
@@ -307,22 +305,25 @@ __device__ void leader_step(const KArgs &a) {
   __shared__ SchedFeatures f;
   __shared__ MlpScratch sc;
   __shared__ uint32_t decide, consumed;
+  __shared__ uint64_t phases_seen;
   Ctrl *c = a.ctrl;
   if (threadIdx.x == 0) {
     decide = 0;
+    // Every control word of the step is loaded in one batch: the step costs
+    // one round trip to the coherence point instead of one per dependent load.
     sched_load(s, &c->s);
-    uint32_t nnear = ld_coh(&c->qcount[s.near_q]);
-    uint32_t nfar = ld_coh(&c->qcount[s.far_q]);
-    uint32_t nheavy = ld_coh(&c->hcount[1u - s.heavy_in]);
+    CtrlCounts k;
+    counts_load(k, c);
+    const uint32_t nnear = k.q(s.near_q), nfar = k.q(s.far_q), nheavy = k.h(1u - s.heavy_in);
     if (nnear > a.qcap || nfar > a.qcap || nheavy > a.hcap) {
       // A push found its queue full (and raised kAbortCapacity); never hand
       // out a count beyond the buffer.
       st_coh(&c->abort, kAbortCapacity);
       s.done = 1;
     } else {
-      stat_add(&c->stat_items, s.in_count);
-      stat_add(&c->stat_heavy, s.heavy_count);
-      stat_add(&c->stat_rounds, 1);
+      st_coh64(&c->stat_items, k.items + s.in_count);
+      st_coh64(&c->stat_heavy, k.heavy + s.heavy_count);
+      st_coh64(&c->stat_rounds, k.rounds + 1);
       if (a.trace && s.round - 1 < a.trace_cap) {
         lsg_round_trace &r = a.trace[s.round - 1];
         r.round = s.round;
@@ -357,16 +358,17 @@ __device__ void leader_step(const KArgs &a) {
         f.curdelta = (float)s.curdelta();
         f.base_delta = a.delta;
         f.prev_delta = s.delta;
-        f.lane_items = (float)ld_coh64(&c->stat_items);
-        f.edges = (float)ld_coh64(&c->stat_edges);
+        f.lane_items = (float)(k.items + s.in_count);
+        f.edges = (float)k.edges;
         f.nnodes = a.nnodes;
         f.nedges = (float)a.nedges;
         f.nsrc = 1;
       } else {
         s.done = 1;
-        stat_add(&c->stat_phases, 1);
+        st_coh64(&c->stat_phases, k.phases + 1);
       }
     }
+    phases_seen = k.phases;
   }
   __syncthreads();
   if (decide) {
@@ -391,7 +393,7 @@ __device__ void leader_step(const KArgs &a) {
       s.set_curdelta(s.curdelta() + s.delta);
       s.phase += 1;
       s.round += 1;
-      stat_add(&c->stat_phases, 1);
+      st_coh64(&c->stat_phases, phases_seen + 1);
     }
   }
   if (threadIdx.x == 0) {
@@ -451,10 +453,9 @@ __global__ void init_kernel(KArgs a, uint32_t delta) {
     a.near_stamp[v] = 0u;
     a.far_stamp[v] = 0u;
   }
+  ctrl_reset_barrier(a.ctrl, tid, nt);
   if (tid == 0) {
     Ctrl *c = a.ctrl;
-    c->bar_count = 0;
-    c->bar_gen = 0;
     c->qcount[0] = 0;
     c->qcount[1] = 0;
     c->qcount[2] = 0;
@@ -619,7 +620,7 @@ static int run_core(Workspace *w, const CsrView &g, const uint32_t *seeds, uint3
       return rc;
     }
     HIPCHK(hipEventRecord(w->ev1, w->stream));
-    HIPCHK(hipMemcpyAsync(&hc, w->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, w->stream));
+    HIPCHK(hipMemcpyAsync(&hc, w->ctrl, kCtrlHostBytes, hipMemcpyDeviceToHost, w->stream));
     HIPCHK(hipStreamSynchronize(w->stream));
     HIPCHK(hipEventElapsedTime(&ms, w->ev0, w->ev1));
     if (hc.abort != kAbortResidency) break;
@@ -633,9 +634,10 @@ static int run_core(Workspace *w, const CsrView &g, const uint32_t *seeds, uint3
   }
   if (hc.abort == kAbortTimeout)
     return fail(LSG_ERR_TIMEOUT,
-                "grid barrier timed out (grid %d x %d) at round %u phase %u done %u with %u of %d blocks "
-                "arrived, generation %u; distances are incomplete",
-                grid, bs, hc.s.round, hc.s.phase, hc.s.done, hc.bar_count, grid, hc.bar_gen);
+                "grid barrier timed out (grid %d x %d) at round %u phase %u done %u with %u of %u arrival "
+                "groups complete, generation %u; distances are incomplete",
+                grid, bs, hc.s.round, hc.s.phase, hc.s.done, hc.bar_count % ((grid + kBarGroup - 1) / kBarGroup),
+                (grid + kBarGroup - 1) / kBarGroup, hc.bar_gen);
   if (hc.abort == kAbortCapacity) return fail(LSG_ERR_CAPACITY, "device queue capacity exceeded");
   if (o.trace) {
     uint32_t n = (uint32_t)std::min<uint64_t>(hc.stat_rounds, kTraceCap);
