@@ -114,14 +114,19 @@ __device__ __forceinline__ void relax_batch(const MArgs &a, const Sched &s, Lane
   for (uint32_t j0 = 0; j0 < n; j0 += kUnroll) {
     uint32_t v[kUnroll], nd[kUnroll], cur[kUnroll], old[kUnroll];
     bool want[kUnroll];
+    // Candidates first (LDS and register work only) ...
 #pragma unroll
     for (int k = 0; k < kUnroll; ++k) {
-      const uint32_t j = min(j0 + k, (uint32_t)kWave - 1);
+      const uint32_t j = min(j0 + k, n - 1);  // a tail slot repeats the batch's last edge, with no candidate
       v[k] = readlane(ev, j);
-      const uint32_t du = j0 + k < n ? dl[readlane(ek, j)][lane] : kInf;
-      nd[k] = du == kInf ? kInf : du + readlane(ew, j);
-      cur[k] = nd[k] != kInf ? ld_coh(a.D + (size_t)v[k] * a.S + lane) : 0u;
+      const uint32_t du = dl[readlane(ek, j)][lane];
+      nd[k] = du == kInf || j0 + k >= n ? kInf : du + readlane(ew, j);
     }
+    // ... then the group's row loads, unconditional so that none waits on a
+    // branch: every lane loads its word of D[v], whose 256-byte row is fetched
+    // anyway when any lane needs it.
+#pragma unroll
+    for (int k = 0; k < kUnroll; ++k) cur[k] = ld_coh(a.D + (size_t)v[k] * a.S + lane);
     // every atomicMin of the group is issued before any result is used
 #pragma unroll
     for (int k = 0; k < kUnroll; ++k) {

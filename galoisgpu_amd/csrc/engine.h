@@ -123,7 +123,7 @@ struct Ctrl {
   uint32_t qcount[3]; uint32_t _p2[29];
   uint32_t hcount[2]; uint32_t _p3[30];
   uint32_t ticket;    uint32_t ticket2; uint32_t _p4[30];
-  uint32_t abort;     uint32_t checkin; uint32_t resident_seen; uint32_t _p5[29];
+  uint32_t abort;     uint32_t checkin; uint32_t resident_seen; uint32_t progress; uint32_t _p5[28];
   Sched s;            uint32_t _p6[16];
   uint64_t stat_items, stat_rounds, stat_heavy, stat_phases, stat_edges;
   uint32_t _p7[22];
@@ -227,42 +227,14 @@ __device__ __forceinline__ bool barrier_arrive(Ctrl *c, uint32_t nblocks) {
   return (atomic_add(&c->bar_count, 1u) + 1) % ngroups == 0;
 }
 
-// Leader grid barrier.  Blocks arrive through barrier_arrive; the last to
-// arrive runs `leader()` (single thread) and bumps the generation; the others
-// spin on the generation.  Replaces GlobalBarrier::Sync
-// (rt/include/gbar.cuh:75-121).  The spin is bounded by `spin_limit` ticks of
-// the constant-rate wall clock; when exceeded, or when another block has
-// aborted, the block raises kAbortTimeout and returns so every wave exits.
-template <class Leader>
-__device__ __forceinline__ void grid_barrier(Ctrl *c, uint32_t nblocks, uint64_t spin_limit, Leader leader) {
-  order_release();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t gen = ld_coh(&c->bar_gen);
-    if (barrier_arrive(c, nblocks)) {
-      order_acquire();
-      leader();
-      order_release();
-      st_coh(&c->bar_gen, gen + 1);
-    } else {
-      uint64_t t0 = wall_clock64();
-      while (ld_coh(&c->bar_gen) == gen) {
-        if (ld_coh(&c->abort) != kAbortNone) break;
-        if (wall_clock64() - t0 > spin_limit) {
-          st_coh(&c->abort, kAbortTimeout);
-          break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-      }
-    }
-  }
-  order_acquire();
-  __syncthreads();
-}
-
-// Same barrier, but every thread of the last-arriving block runs `leader()`
-// (which may use __syncthreads and LDS), so leader work such as scheduler
-// inference can be spread over the block.
+// Leader grid barrier.  Blocks arrive through barrier_arrive; every thread of
+// the last block to arrive runs `leader()` (which may use __syncthreads and
+// LDS, so leader work such as scheduler inference is spread over the block)
+// and then bumps the generation; the others spin on the generation.  Replaces
+// GlobalBarrier::Sync (rt/include/gbar.cuh:75-121).  The spin is bounded by
+// `spin_limit` ticks of the constant-rate wall clock without progress; when
+// exceeded, or when another block has aborted, the block raises kAbortTimeout
+// and returns so every wave exits.
 template <class Leader>
 __device__ __forceinline__ void grid_barrier_block(Ctrl *c, uint32_t nblocks, uint64_t spin_limit, Leader leader) {
   __shared__ uint32_t s_lead, s_gen;
@@ -283,9 +255,18 @@ __device__ __forceinline__ void grid_barrier_block(Ctrl *c, uint32_t nblocks, ui
   } else if (threadIdx.x == 0) {
     uint32_t gen = s_gen;
     uint64_t t0 = wall_clock64();
+    uint32_t seen = ld_coh(&c->progress);
     while (ld_coh(&c->bar_gen) == gen) {
       if (ld_coh(&c->abort) != kAbortNone) break;
       if (wall_clock64() - t0 > spin_limit) {
+        // The leader block may be running rounds on its own (it bumps
+        // `progress` each round): keep waiting while it advances.
+        uint32_t p = ld_coh(&c->progress);
+        if (p != seen) {
+          seen = p;
+          t0 = wall_clock64();
+          continue;
+        }
         st_coh(&c->abort, kAbortTimeout);
         break;
       }

@@ -402,6 +402,48 @@ __device__ void leader_step(const KArgs &a) {
   }
 }
 
+// Rounds small enough for one block.  While the next round's in-queue fits in
+// kLocalTiles tiles and has no heavy chunks, the block that ran the leader
+// step runs that round itself, separated by block barriers, while the rest of
+// the grid stays parked at the grid barrier; it bumps `progress` each round so
+// the parked blocks' spin limit measures lack of progress.  A round of a small
+// frontier (high-diameter graphs, the first and last rounds of any traversal)
+// then costs a few dependent memory round trips instead of a grid barrier.
+#ifndef LSG_LOCAL_TILES
+#define LSG_LOCAL_TILES 1
+#endif
+constexpr uint32_t kLocalTiles = LSG_LOCAL_TILES;
+
+template <Algo A, int BS>
+__device__ void local_rounds(const KArgs &a, TileShared<BS> &sh) {
+  __shared__ Sched ls;
+  __shared__ uint32_t go;
+  Ctrl *c = a.ctrl;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      order_release();  // the schedule this thread just stored is complete before it is read back
+      sched_load(ls, &c->s);
+      go = !ls.done && ls.heavy_count == 0 && ls.in_count <= kLocalTiles * BS &&
+           ld_coh(&c->abort) == kAbortNone;
+      sh.edges = 0;
+    }
+    __syncthreads();
+    if (!go) return;
+    const Sched s = ls;
+    for (uint32_t t = 0; t * BS < s.in_count; ++t) {
+      process_tile<A, BS>(a, s, t, sh);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0 && (a.sched || a.feat) && sh.edges)
+      __hip_atomic_fetch_add(&c->stat_edges, (uint64_t)sh.edges, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // this block's pushes and counts are complete before the leader step reads them
+    order_release();
+    __syncthreads();
+    leader_step<A>(a);
+    if (threadIdx.x == 0) st_coh(&c->progress, s.round);
+  }
+}
+
 template <Algo A, int BS>
 __global__ __launch_bounds__(BS) void persistent_kernel(KArgs a) {
   __shared__ TileShared<BS> sh;
@@ -440,7 +482,10 @@ __global__ __launch_bounds__(BS) void persistent_kernel(KArgs a) {
       if (threadIdx.x == 0 && sh.edges)
         __hip_atomic_fetch_add(&c->stat_edges, (uint64_t)sh.edges, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    grid_barrier_block(c, nb, a.spin_limit, [&] { leader_step<A>(a); });
+    grid_barrier_block(c, nb, a.spin_limit, [&] {
+      leader_step<A>(a);
+      local_rounds<A, BS>(a, sh);
+    });
   }
 }
 

@@ -219,9 +219,13 @@ def sweep_instance(dev, srcs, base_delta: int, model: np.ndarray, max_greedy_pha
 def training_instances(quick: bool = False, kind: str = "batched") -> List[dict]:
     """Graphs the scheduler is trained on (none of them is a benchmark graph)."""
     if kind == "plain":
-        inst = [dict(kind="road", rows=384, cols=384, seed=21, nsrc=1, delta=10000),
-                dict(kind="road", rows=512, cols=512, seed=22, nsrc=1, delta=2000),
-                dict(kind="road", rows=768, cols=768, seed=23, nsrc=1, delta=10000),
+        # road-like grids across sizes and delta widths, plain grids, RMAT; the
+        # benchmark graphs (seed 1) are not among them
+        inst = [dict(kind="road", rows=1024, cols=1024, seed=21, nsrc=1, delta=10000),
+                dict(kind="road", rows=1536, cols=1536, seed=22, nsrc=1, delta=10000),
+                dict(kind="road", rows=2048, cols=1024, seed=23, nsrc=1, delta=5000),
+                dict(kind="road", rows=1024, cols=1024, seed=29, nsrc=1, delta=50000),
+                dict(kind="road", rows=512, cols=512, seed=30, nsrc=1, delta=2000),
                 dict(kind="grid", rows=512, cols=512, seed=24, nsrc=1, delta=32),
                 dict(kind="grid", rows=256, cols=1024, seed=25, nsrc=1, delta=100),
                 dict(kind="rmat", scale=16, seed=26, nsrc=1, delta=32),
