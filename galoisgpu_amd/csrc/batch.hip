@@ -110,6 +110,9 @@ __device__ __forceinline__ void relax_batch(const MArgs &a, const Sched &s, Lane
   // Near/far threshold of this lane: the shared curdelta (apps/sssp/kernel.cu:134)
   // or, in a bucket phase, the lane's own source threshold.
   const int64_t thr = ls.bucket ? (int64_t)ls.thr : s.curdelta();
+  // Column this lane loads: its own source, or for lanes >= S (no source, no
+  // candidate) the row's last word, so that every load stays inside row v.
+  const uint32_t col = min(lane, a.S - 1);
   uint64_t my_nb = 0, my_fb = 0;
   for (uint32_t j0 = 0; j0 < n; j0 += kUnroll) {
     uint32_t v[kUnroll], nd[kUnroll], cur[kUnroll], old[kUnroll];
@@ -123,10 +126,10 @@ __device__ __forceinline__ void relax_batch(const MArgs &a, const Sched &s, Lane
       nd[k] = du == kInf || j0 + k >= n ? kInf : du + readlane(ew, j);
     }
     // ... then the group's row loads, unconditional so that none waits on a
-    // branch: every lane loads its word of D[v], whose 256-byte row is fetched
+    // branch: every lane loads a word of D[v], whose row (S words) is fetched
     // anyway when any lane needs it.
 #pragma unroll
-    for (int k = 0; k < kUnroll; ++k) cur[k] = ld_coh(a.D + (size_t)v[k] * a.S + lane);
+    for (int k = 0; k < kUnroll; ++k) cur[k] = ld_coh(a.D + (size_t)v[k] * a.S + col);
     // every atomicMin of the group is issued before any result is used
 #pragma unroll
     for (int k = 0; k < kUnroll; ++k) {

@@ -82,6 +82,25 @@ def test_multi_partial_width_and_batch(rmat16):
     dev.close()
 
 
+def test_partial_width_rows_up_to_the_last_vertex():
+    """Fewer than 64 sources on a graph whose last vertex is reached: rows of
+    D are S words, and relaxing edges into the last row exercises the row
+    loads at the end of D (the grid's corner vertex is every source's
+    destination)."""
+    g = CSRGraph.grid(96, 80, 1, 100, seed=8)
+    V = g.nnodes
+    srcs = np.random.default_rng(8).integers(0, V, 64)
+    srcs[0] = V - 1
+    ref = np.stack([oracle.dijkstra(g.row_start, g.edge_dst, g.edge_data, int(s)) for s in srcs], axis=1)
+    dev = g.copy_to_gpu()
+    for S in (1, 7, 32, 64):
+        for far in (0, 2):
+            say("width to last vertex", S, far)
+            d, _ = sssp_multi(dev, srcs[:S], Options(delta=64, far_mode=far))
+            assert np.array_equal(d, ref[:, :S]), (S, far)
+    dev.close()
+
+
 def test_forced_schedules_keep_distances(rmat16):
     """Distances are identical whatever schedule the scheduler imposes."""
     g, srcs, ref = rmat16
