@@ -31,17 +31,18 @@
 
 namespace lsg {
 
+// Tuning constants, measured on RMAT-22 x 64 sources (ms per launch): 16 row
+// loads and 16 items per ticket at 4 waves/SIMD 65-66; 8 / 8 at 5 waves 68;
+// 8 / 8 at 6 waves 68-69; 4 / 8 at 6 waves 69-70.
 #ifndef LSG_MULTI_UNROLL
-#define LSG_MULTI_UNROLL 8
+#define LSG_MULTI_UNROLL 16
 #endif
 #ifndef LSG_MULTI_ITEMS
-#define LSG_MULTI_ITEMS 8
+#define LSG_MULTI_ITEMS 16
 #endif
 // Waves per SIMD the register allocation must allow (0: compiler's choice).
-// 5 trades a few spills for a fifth resident block per CU (measured on
-// RMAT-22 x 64 sources: 72 ms at 5, 77 ms at the compiler's 4).
 #ifndef LSG_MULTI_WAVES
-#define LSG_MULTI_WAVES 5
+#define LSG_MULTI_WAVES 4
 #endif
 // Row loads issued ahead per wavefront in the edge loop.
 constexpr int kUnroll = LSG_MULTI_UNROLL;

@@ -245,3 +245,27 @@ def test_cli_output_matches_oracle_text(tmp_path):
     lv, _ = oracle.bfs(g.row_start, g.edge_dst, 5)
     want = "".join(("%d INF\n" % i) if x == INF else ("%d %d\n" % (i, x)) for i, x in enumerate(lv))
     assert open(out2).read() == want
+
+
+def test_bmk_harness_runs_the_cli(tmp_path):
+    """The bmk2-style harness runs bin/sssp and bin/bfs, parses Total time and
+    diff-checks the output against a reference written from the oracle."""
+    from galoisgpu_amd import bmk
+    g = CSRGraph.grid(48, 40, 1, 100, seed=12)
+    path = str(tmp_path / "g.gr")
+    g.write(path)
+    ref = oracle.dijkstra(g.row_start, g.edge_dst, g.edge_data, 5)
+    apps.output(ref, str(tmp_path / "g.sssp.out"))
+    lvl, _ = oracle.bfs(g.row_start, g.edge_dst, 5)
+    apps.output(lvl, str(tmp_path / "g.bfs.out"))
+    say("bmk sssp")
+    r = bmk.run("sssp", path, delta=32, source=5, runs=2)
+    assert r["check"] == "pass", r
+    assert len(r["times_ns"]) == 2 and all(t and t > 0 for t in r["times_ns"])
+    say("bmk bfs")
+    assert bmk.run("bfs", path, source=5)["check"] == "pass"
+    bad = str(tmp_path / "bad.out")
+    ref[7] += 1
+    apps.output(ref, bad)
+    r = bmk.run("sssp", path, delta=32, source=5, ref=bad)
+    assert r["check"] == "fail" and r["line"] == 8
