@@ -72,7 +72,7 @@ def main() -> int:
     tuned_b = autotune.tuned_options(tname, "batched") or {}
     tuned_p = autotune.tuned_options(tname, "plain") or {}
     launch_b = {k: v for k, v in tuned_b.items() if k in ("heavy_deg", "block_size")}
-    launch_p = {k: v for k, v in tuned_p.items() if k in ("expand_mode", "block_size")}
+    launch_p = {k: v for k, v in tuned_p.items() if k in ("expand_mode", "heavy_deg", "block_size")}
     use_learned = False
     model_path = lsched.default_model_path("batched")
     if args.sched == "learned" or (args.sched == "auto" and os.path.exists(model_path)):

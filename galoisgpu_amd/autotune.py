@@ -36,6 +36,7 @@ BATCH_SPACE = {
 PLAIN_SPACE = {
     "delta": [16, 32, 64, 256],
     "expand_mode": [0, 1],
+    "heavy_deg": [256, 512, 4096],
     "block_size": [256, 512],
 }
 
