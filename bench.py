@@ -42,7 +42,7 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-plain", action="store_true", help="skip timing the plain kernel")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-sample", type=int, default=8, help="sources in the CPU baseline sample")
+    ap.add_argument("--cpu-sample", type=int, default=16, help="sources in the CPU baseline sample")
     ap.add_argument("--sched", choices=["auto", "fixed", "learned"], default="auto")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -151,23 +151,31 @@ def main() -> int:
         log(f"plain: {plain['ms_per_step']:.3f} ms/step wall, {plain['gteps']:.2f} GTEPS")
 
     # ---- CPU baseline: the oracle's restatement of the reference near/far ---
+    # One source per host thread (the C restatement runs with the GIL released),
+    # at most 16 threads: the host share of one GPU.
     cpu = None
     if not args.no_cpu:
         import oracle
+        from concurrent.futures import ThreadPoolExecutor
         sample = [int(s) for s, e in zip(srcs, er) if e > 0][: args.cpu_sample]
+        threads = max(1, min(len(sample), os.cpu_count() or 1, 16))
         w = g.edge_data
-        tc = time.perf_counter()
-        ce = 0
-        for s in sample:
+
+        def one(s):
             d, _ = oracle.nearfar(g.row_start, g.edge_dst, w, s, args.delta)
             k = int(np.flatnonzero(srcs == s)[0])
-            assert np.array_equal(d, Dh[:, k]), "CPU restatement disagrees with the GPU"
-            ce += int(er[k])
+            return bool(np.array_equal(d, Dh[:, k])), int(er[k])
+
+        tc = time.perf_counter()
+        with ThreadPoolExecutor(max_workers=threads) as ex:
+            res = list(ex.map(one, sample))
         tcpu = time.perf_counter() - tc
-        cpu = {"value": ce / tcpu / 1e9, "unit": "GTEPS", "cores": 1, "kind": "port",
-               "sample": f"{len(sample)} of the {args.nsrc} sources (those with E_r>0), serial near/far "
-                         f"restatement of apps/sssp (oracle/oracle.c), {tcpu:.1f}s"}
-        log(f"cpu: {cpu['value']:.4f} GTEPS on {len(sample)} sources in {tcpu:.1f}s")
+        assert all(ok for ok, _ in res), "CPU restatement disagrees with the GPU"
+        ce = sum(e for _, e in res)
+        cpu = {"value": ce / tcpu / 1e9, "unit": "GTEPS", "cores": threads, "kind": "port",
+               "sample": f"{len(sample)} of the {args.nsrc} sources (those with E_r>0), one per thread, serial "
+                         f"near/far restatement of apps/sssp (oracle/oracle.c), {tcpu:.1f}s wall"}
+        log(f"cpu: {cpu['value']:.4f} GTEPS on {len(sample)} sources with {threads} threads in {tcpu:.1f}s")
 
     traffic = lsched.profiled_traffic(ROOT, "sssp_multi_rmat22")
     out = {
