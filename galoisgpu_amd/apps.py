@@ -24,7 +24,7 @@ class Options:
     delta: int = 10000          # -d, apps/sssp/support.cu:10
     start_node: int = 0         # -s, apps/sssp/support.cu:12
     enable_lb: bool = False     # -l
-    expand_mode: int = EXPAND_TWB
+    expand_mode: int = EXPAND_LB
     far_mode: int = FAR_REFERENCE
     block_size: int = 0
     grid_blocks: int = 0

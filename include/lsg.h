@@ -58,8 +58,8 @@ typedef struct lsg_graph {
 enum {
   LSG_EXPAND_TWB = 0, /* by degree: thread (< thread_deg), wavefront (< heavy_deg),
                          heavy vertices split into block chunks */
-  LSG_EXPAND_LB = 1   /* tile-flattened: block prefix sum of degrees, every lane
-                         takes one edge (the reference's fine-grained np) */
+  LSG_EXPAND_LB = 1   /* tile-flattened (default): block prefix sum of degrees, every
+                         lane takes one edge (the reference's fine-grained np) */
 };
 
 /* Far-pile modes.  Every mode gives the same distances; they differ in which

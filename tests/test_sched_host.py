@@ -30,7 +30,8 @@ def test_plain_model_action_tables():
     for a, (mult, mode) in enumerate(lsched.ACTIONS_BY_KIND["plain"]):
         assert m[lsched.OFF_DELTA + a] == np.float32(mult)
         assert m[lsched.OFF_FAR + a] == mode  # expansion mode: 0 TWB, 1 LB
-    assert lsched.ACTIONS_BY_KIND["plain"][lsched.FIXED_ACTION] == (1.0, 0)  # delta x1, degree-binned
+    # delta x1 with the default, tile-flattened expansion
+    assert lsched.ACTIONS_BY_KIND["plain"][lsched.FIXED_BY_KIND["plain"]] == (1.0, 1)
     assert sorted({mode for _, mode in lsched.ACTIONS_BY_KIND["plain"]}) == [0, 1]
 
 
