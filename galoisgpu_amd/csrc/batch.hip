@@ -80,6 +80,7 @@ struct LaneSched {
   uint32_t thr;   // near/far threshold of this lane's source
   bool bucket;    // per-source thresholds (LSG_FAR_BUCKET phase)
   uint32_t fmin;  // smallest distance this lane pushed to the far pile this round
+  uint32_t thr32; // threshold of this lane as u32: a candidate nd goes near iff nd <= thr32
 };
 
 // Counters the leader turns into trace records and scheduler features.
@@ -158,6 +159,115 @@ __device__ __forceinline__ void relax_batch(const MArgs &a, const Sched &s, Lane
   wave_push(pf, ev, a.q[s.far_q], &a.ctrl->qcount[s.far_q], a.qcap, &a.ctrl->abort);
 }
 
+// ---- item-major relaxation ------------------------------------------------
+//
+// Items of degree >= kLongDeg and heavy chunks are relaxed one item at a time:
+// the item's distance for this lane's source stays in a register, its edges
+// come 16 at a time through the scalar cache (one s_load_dwordx16 for the
+// destinations, one for the weights), and each edge costs one row load, an
+// add and a compare per wavefront -- no lane shuffles.  Improvements are
+// collected in a per-wavefront list and the destinations' masks are updated
+// 64 at a time.
+constexpr int kGroup = 16;
+constexpr uint32_t kLongDeg = kGroup;  // a relax window must fit inside the item
+
+typedef uint32_t U32x16 __attribute__((ext_vector_type(16), aligned(4)));
+typedef const __attribute__((address_space(4))) U32x16 ScalarX16;
+typedef const __attribute__((address_space(4))) uint32_t ScalarU32;
+
+// Words [e, e + 16) of a CSR array that the launch never writes, loaded into
+// scalar registers with one instruction (e is wave-uniform).
+__device__ __forceinline__ U32x16 scalar_x16(const uint32_t *arr, uint32_t e) {
+  return *(ScalarX16 *)(arr + e);
+}
+
+// Improved edges since the last flush: lane i holds the destination of the
+// i-th and the sources for which it must join the near / far pile.
+struct Pending {
+  uint32_t v, nb_lo, nb_hi, fb_lo, fb_hi;
+  uint32_t n;  // wave-uniform
+};
+
+
+// Sets the pending destinations' mask bits; a destination whose mask was
+// empty joins the pile's queue (two entries for one destination: only the
+// lane whose OR finds the mask empty pushes it).
+__device__ __forceinline__ void flush_pending(const MArgs &a, const Sched &s, Pending &p) {
+  if (p.n == 0) return;
+  const bool act = lane_id() < p.n;
+  const uint64_t nb = ((uint64_t)p.nb_hi << 32) | p.nb_lo, fb = ((uint64_t)p.fb_hi << 32) | p.fb_lo;
+  const bool pn = act && nb != 0 && atomic_or64(a.nmask[s.round & 1] + p.v, nb) == 0;
+  const bool pf = act && fb != 0 && atomic_or64(a.fmask[s.phase & 1] + p.v, fb) == 0;
+  wave_push(pn, p.v, a.q[s.near_q], &a.ctrl->qcount[s.near_q], a.qcap, &a.ctrl->abort);
+  wave_push(pf, p.v, a.q[s.far_q], &a.ctrl->qcount[s.far_q], a.qcap, &a.ctrl->abort);
+  p.n = 0;
+}
+
+// Relaxes the 16 edges with destinations vv and weights ww leaving an item
+// whose distance for this lane's source is du (INF: no candidate); the first
+// `skip` of them carry no candidate (relaxed already).  The 16 row loads are
+// issued back to back, then the atomicMins of the lanes whose candidate beats
+// the loaded distance.
+__device__ __forceinline__ void relax_group(const MArgs &a, LaneSched &ls, uint32_t du, const U32x16 &vv,
+                                            const U32x16 &ww, uint32_t skip, Pending &p, uint64_t &improved) {
+  const uint32_t lane = lane_id();
+  // lanes >= S have no source and du = INF: they load the row's last word
+  // and never issue an atomic
+  const uint32_t col = min(lane, a.S - 1);
+  uint32_t nd[kGroup], cur[kGroup];
+#pragma unroll
+  for (int k = 0; k < kGroup; ++k) {
+    // du + w never wraps (du <= INF, w <= INT_MAX); INF + w and a skipped
+    // slot's du + 2^31 both exceed every stored distance
+    nd[k] = du + ((uint32_t)k < skip ? 0x80000000u : ww[k]);
+  }
+#pragma unroll
+  for (int k = 0; k < kGroup; ++k) cur[k] = ld_coh(a.D + (size_t)vv[k] * a.S + col);
+#pragma unroll
+  for (int k = 0; k < kGroup; ++k)
+    cur[k] = nd[k] < cur[k] ? atomic_min(a.D + (size_t)vv[k] * a.S + col, nd[k]) : 0u;
+  // Results are taken only here, after every atomic of the group has been
+  // issued (left to itself the compiler tests each result inside its branch,
+  // waiting for the whole memory queue after every atomic).
+#pragma unroll
+  for (int k = 0; k < kGroup; ++k) asm volatile("" : "+v"(cur[k]));
+#pragma unroll
+  for (int k = 0; k < kGroup; ++k) {
+    const bool imp = nd[k] < cur[k];
+    const uint64_t im = __ballot(imp);
+    if (im == 0) continue;
+    const bool nearl = imp && nd[k] <= ls.thr32;
+    const uint64_t nb = __ballot(nearl), fb = im & ~nb;
+    if (fb && imp && !nearl) ls.fmin = min(ls.fmin, nd[k]);
+    improved += 1;
+    if (lane == p.n) {
+      p.v = vv[k];
+      p.nb_lo = (uint32_t)nb;
+      p.nb_hi = (uint32_t)(nb >> 32);
+      p.fb_lo = (uint32_t)fb;
+      p.fb_hi = (uint32_t)(fb >> 32);
+    }
+    p.n += 1;
+  }
+}
+
+// Relaxes edges [lo, hi) of one item with at least 16 edges (distance du for
+// this lane's source), 16 at a time; the last window is shifted back to end
+// at hi, so every window lies inside the item's edges, and its slots that
+// the previous window covered are skipped.
+__device__ __forceinline__ void relax_edges(const MArgs &a, const Sched &s, LaneSched &ls, uint32_t du, uint32_t lo,
+                                            uint32_t hi, Pending &p, uint64_t &improved) {
+  // the bounds are wave-uniform; say so, so that they stay in scalar registers
+  lo = __builtin_amdgcn_readfirstlane(lo);
+  hi = __builtin_amdgcn_readfirstlane(hi);
+  for (uint32_t e = lo; e < hi; e += kGroup) {
+    if (p.n > (uint32_t)(kWave - kGroup)) flush_pending(a, s, p);
+    const uint32_t w0 = min(e, hi - kGroup);
+    const U32x16 vv = scalar_x16(a.edge_dst, w0), ww = scalar_x16(a.edge_data, w0);
+    relax_group(a, ls, du, vv, ww, e - w0, p, improved);
+  }
+}
+
 // Queues heavy vertex u (edges [rs, re)) as chunks of heavy_deg edges, each
 // expanded by a wavefront in the next round with the sources active now.
 __device__ __forceinline__ void queue_heavy(const MArgs &a, const Sched &s, uint32_t u, uint32_t rs, uint32_t re,
@@ -184,7 +294,7 @@ __device__ __forceinline__ void queue_heavy(const MArgs &a, const Sched &s, uint
 // loaded together and parked in dl, and the items' edges are relaxed as one
 // flattened list, 64 per batch.  Heavy items are queued as chunks instead.
 __device__ __forceinline__ void process_items(const MArgs &a, const Sched &s, LaneSched &ls, uint32_t base,
-                                              DistSlots &dl, MultiCounters &mc) {
+                                              DistSlots &dl, MultiCounters &mc, Pending &p) {
   const uint32_t lane = lane_id();
   const uint32_t cnt = min((uint32_t)kItems, s.in_count - base);
   uint64_t *mp = s.aux[0] ? a.fmask[(s.phase - 1) & 1] : a.nmask[(s.round - 1) & 1];
@@ -207,7 +317,8 @@ __device__ __forceinline__ void process_items(const MArgs &a, const Sched &s, La
     du[k] = lane < a.S && ((mk >> lane) & 1) ? ld_coh(a.D + (size_t)uk * a.S + lane) : kInf;
   }
   const bool split = s.aux[0] && s.aux[1];
-  uint32_t mydeg = 0;  // lane k: edges of item k to relax in this wavefront
+  uint32_t mydeg = 0;  // lane k: edges of item k to relax in the flattened list
+  uint32_t longm = 0;  // items relaxed one by one (degree >= kLongDeg)
 #pragma unroll
   for (int k = 0; k < kItems; ++k) {
     bool act = du[k] != kInf;
@@ -238,7 +349,8 @@ __device__ __forceinline__ void process_items(const MArgs &a, const Sched &s, La
     dl[k][lane] = act ? du[k] : kInf;
     mc.lane_items += __builtin_popcountll(am);
     mc.edges += rek - rsk;
-    if (lane == (uint32_t)k) mydeg = rek - rsk;
+    if (rek - rsk >= kLongDeg) longm |= 1u << k;
+    else if (lane == (uint32_t)k) mydeg = rek - rsk;
   }
   // Inclusive prefix of the items' edge counts over lanes 0..kItems-1: item k
   // owns flattened edges [end_k - deg_k, end_k).
@@ -264,12 +376,16 @@ __device__ __forceinline__ void process_items(const MArgs &a, const Sched &s, La
     }
     relax_batch(a, s, ls, dl, min((uint32_t)kWave, total - b0), ev, ew, item, &mc.improved);
   }
+  for (uint32_t lm = longm; lm; lm &= lm - 1) {
+    const uint32_t k = (uint32_t)__builtin_ctz(lm);
+    relax_edges(a, s, ls, dl[k][lane], readlane(rs, k), readlane(re, k), p, mc.improved);
+  }
 }
 
 // Heavy chunk c: edges [k * heavy_deg, (k + 1) * heavy_deg) of u, for the
-// sources recorded with the chunk; the distance row goes to slot 0.
+// sources recorded with the chunk.
 __device__ __forceinline__ void process_chunk(const MArgs &a, const Sched &s, LaneSched &ls, uint32_t c,
-                                              DistSlots &dl, MultiCounters &mc) {
+                                              MultiCounters &mc, Pending &p) {
   const uint32_t lane = lane_id();
   uint64_t ent = ld_coh64(a.hq[s.heavy_in] + 2ull * c);
   uint64_t m = ld_coh64(a.hq[s.heavy_in] + 2ull * c + 1);
@@ -285,15 +401,7 @@ __device__ __forceinline__ void process_chunk(const MArgs &a, const Sched &s, La
   if (am == 0) return;
   mc.lane_items += __builtin_popcountll(am);
   mc.edges += hi - lo;
-  dl[0][lane] = act ? du : kInf;
-  for (uint32_t b = lo; b < hi; b += kWave) {
-    uint32_t ev = 0, ew = 0;
-    if (b + lane < hi) {
-      ev = a.edge_dst[b + lane];
-      ew = a.edge_data[b + lane];
-    }
-    relax_batch(a, s, ls, dl, min((uint32_t)kWave, hi - b), ev, ew, 0u, &mc.improved);
-  }
+  relax_edges(a, s, ls, act ? du : kInf, lo, hi, p, mc.improved);
 }
 
 // Leader step, run by every thread of the last block to reach the barrier:
@@ -451,7 +559,11 @@ void multi_kernel(MArgs a, MultiCounters *tot) {
     LaneSched ls;
     ls.bucket = s.aux[1] == LSG_FAR_BUCKET;
     ls.thr = ls.bucket && lane_id() < a.S ? ld_coh(a.thr + lane_id()) : 0u;
+    ls.thr32 = ls.bucket ? ls.thr : (uint32_t)min(s.curdelta(), (int64_t)UINT32_MAX);
     ls.fmin = kInf;
+    Pending p;
+    p.n = 0;
+    p.v = p.nb_lo = p.nb_hi = p.fb_lo = p.fb_hi = 0;
     DistSlots &dl = dls[threadIdx.x / kWave];
     // Static first pass (ticket = wavefront id), then dynamic tickets; the
     // next ticket is fetched while the current one is processed.
@@ -459,15 +571,16 @@ void multi_kernel(MArgs a, MultiCounters *tot) {
     for (uint32_t ck = wave; ck < nchunks;) {
       uint32_t t = 0;
       if (lane_id() == 0) t = atomic_add(&c->ticket, 1u);
-      process_items(a, s, ls, ck * kItems, dl, mc);
+      process_items(a, s, ls, ck * kItems, dl, mc, p);
       ck = nwaves + readlane(t, 0);
     }
     for (uint32_t ck = wave; ck < s.heavy_count;) {
       uint32_t t = 0;
       if (lane_id() == 0) t = atomic_add(&c->ticket2, 1u);
-      process_chunk(a, s, ls, ck, dl, mc);
+      process_chunk(a, s, ls, ck, mc, p);
       ck = nwaves + readlane(t, 0);
     }
+    flush_pending(a, s, p);
     // this wavefront's smallest far push per source -> phase minimum
     if (__ballot(ls.fmin != kInf) && ls.fmin != kInf) atomic_min(a.fmin + (s.phase & 1) * 64 + lane_id(), ls.fmin);
     if (lane_id() == 0 && (mc.lane_items | mc.edges)) {
@@ -578,6 +691,7 @@ static int run_multi(Workspace *w, const lsg_graph *g, const uint32_t *srcs_host
   int bs = o.block_size ? o.block_size : 512;
   if (bs != 256 && bs != 512 && bs != 1024) return fail(LSG_ERR_ARG, "block_size must be 256, 512 or 1024");
   uint32_t heavy_deg = o.heavy_deg > 0 ? (uint32_t)o.heavy_deg : 2048u;
+  if (heavy_deg < kLongDeg) return fail(LSG_ERR_ARG, "heavy_deg must be >= %u for batched runs", kLongDeg);
   uint32_t V = g->nnodes;
   uint64_t hcap64 = 2ull * g->nedges / heavy_deg + 64;
   uint32_t hcap = (uint32_t)std::min<uint64_t>(hcap64, UINT32_MAX / 4);
