@@ -28,9 +28,9 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 BATCH_SPACE = {
-    "delta": [32, 128, 512, 1 << 30],
+    "delta": [32, 256, 1024, 1 << 30],
     "far_mode": [0, 2],
-    "heavy_deg": [128, 256, 1024],
+    "heavy_deg": [256, 512, 1024],
     "block_size": [256, 512],
 }
 PLAIN_SPACE = {

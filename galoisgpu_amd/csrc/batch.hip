@@ -31,14 +31,20 @@
 
 namespace lsg {
 
-// Tuning constants, measured on RMAT-22 x 64 sources (ms per launch): 16 row
-// loads and 16 items per ticket at 4 waves/SIMD 65-66; 8 / 8 at 5 waves 68;
-// 8 / 8 at 6 waves 68-69; 4 / 8 at 6 waves 69-70.
+// Flattened path (items of degree < kLongDeg): row loads issued ahead per
+// wavefront and frontier items per ticket.  Measured on RMAT-22 x 64 sources
+// when every item took this path (ms per launch): 16 / 16 at 4 waves/SIMD
+// 65-66; 8 / 8 at 5 waves 68; 8 / 8 at 6 waves 68-69; 4 / 8 at 6 waves 69-70.
 #ifndef LSG_MULTI_UNROLL
 #define LSG_MULTI_UNROLL 16
 #endif
 #ifndef LSG_MULTI_ITEMS
 #define LSG_MULTI_ITEMS 16
+#endif
+// Gauss-Seidel pops: a popped vertex also takes the near bits set for the
+// next round so far (1) or only its own round's bits (0).
+#ifndef LSG_STEAL
+#define LSG_STEAL 1
 #endif
 // Waves per SIMD the register allocation must allow (0: compiler's choice).
 #ifndef LSG_MULTI_WAVES
@@ -73,6 +79,7 @@ struct MArgs {
   float *feat;          // per-phase feature records [kMaxForced][kFeatRec], or null
   uint32_t *fmin;       // [2][64] smallest pending far distance per source, by phase parity
   uint32_t *thr;        // [64] per-source near/far threshold of the current phase (bucket mode)
+  uint32_t *qstamp;     // [V] last round whose in-queue holds the vertex
 };
 
 // Per-round, per-lane view of the schedule used by the edge loop.
@@ -93,6 +100,13 @@ struct MultiCounters {
 __device__ __forceinline__ uint64_t bcast64(uint64_t x, uint32_t l) {
   uint32_t lo = readlane((uint32_t)x, l), hi = readlane((uint32_t)(x >> 32), l);
   return ((uint64_t)hi << 32) | lo;
+}
+
+// Near-pile push admission: a vertex joins the next round's queue at most
+// once (a pop that took its next-round bits leaves it queued while its mask
+// is empty again).
+__device__ __forceinline__ bool near_admit(const MArgs &a, const Sched &s, uint32_t v) {
+  return __hip_atomic_fetch_max(a.qstamp + v, s.round + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < s.round + 1;
 }
 
 // Distance rows of the items a wavefront is expanding: dl[slot][lane] is the
@@ -153,7 +167,7 @@ __device__ __forceinline__ void relax_batch(const MArgs &a, const Sched &s, Lane
   }
   // Two edges of a batch may share a destination; only the lane whose OR
   // finds the mask empty queues it.
-  bool pn = my_nb != 0 && atomic_or64(a.nmask[s.round & 1] + ev, my_nb) == 0;
+  bool pn = my_nb != 0 && atomic_or64(a.nmask[s.round & 1] + ev, my_nb) == 0 && near_admit(a, s, ev);
   bool pf = my_fb != 0 && atomic_or64(a.fmask[s.phase & 1] + ev, my_fb) == 0;
   wave_push(pn, ev, a.q[s.near_q], &a.ctrl->qcount[s.near_q], a.qcap, &a.ctrl->abort);
   wave_push(pf, ev, a.q[s.far_q], &a.ctrl->qcount[s.far_q], a.qcap, &a.ctrl->abort);
@@ -163,22 +177,27 @@ __device__ __forceinline__ void relax_batch(const MArgs &a, const Sched &s, Lane
 //
 // Items of degree >= kLongDeg and heavy chunks are relaxed one item at a time:
 // the item's distance for this lane's source stays in a register, its edges
-// come 16 at a time through the scalar cache (one s_load_dwordx16 for the
+// come kGroup at a time through the scalar cache (one s_load_dwordx8 for the
 // destinations, one for the weights), and each edge costs one row load, an
-// add and a compare per wavefront -- no lane shuffles.  Improvements are
+// add and a compare per wavefront -- no lane shuffles.  Window width,
+// measured on RMAT-22 x 64 sources: 8 wide and software-pipelined 33.6 ms,
+// 16 wide 34.2 (pipelined or not; pipelined it spills).  Improvements are
 // collected in a per-wavefront list and the destinations' masks are updated
 // 64 at a time.
-constexpr int kGroup = 16;
+#ifndef LSG_GROUP
+#define LSG_GROUP 8
+#endif
+constexpr int kGroup = LSG_GROUP;
 constexpr uint32_t kLongDeg = kGroup;  // a relax window must fit inside the item
 
-typedef uint32_t U32x16 __attribute__((ext_vector_type(16), aligned(4)));
-typedef const __attribute__((address_space(4))) U32x16 ScalarX16;
+typedef uint32_t U32xG __attribute__((ext_vector_type(kGroup), aligned(4)));
+typedef const __attribute__((address_space(4))) U32xG ScalarXG;
 typedef const __attribute__((address_space(4))) uint32_t ScalarU32;
 
-// Words [e, e + 16) of a CSR array that the launch never writes, loaded into
-// scalar registers with one instruction (e is wave-uniform).
-__device__ __forceinline__ U32x16 scalar_x16(const uint32_t *arr, uint32_t e) {
-  return *(ScalarX16 *)(arr + e);
+// Words [e, e + kGroup) of a CSR array that the launch never writes, loaded
+// into scalar registers with one instruction (e is wave-uniform).
+__device__ __forceinline__ U32xG scalar_window(const uint32_t *arr, uint32_t e) {
+  return *(ScalarXG *)(arr + e);
 }
 
 // Improved edges since the last flush: lane i holds the destination of the
@@ -196,52 +215,64 @@ __device__ __forceinline__ void flush_pending(const MArgs &a, const Sched &s, Pe
   if (p.n == 0) return;
   const bool act = lane_id() < p.n;
   const uint64_t nb = ((uint64_t)p.nb_hi << 32) | p.nb_lo, fb = ((uint64_t)p.fb_hi << 32) | p.fb_lo;
-  const bool pn = act && nb != 0 && atomic_or64(a.nmask[s.round & 1] + p.v, nb) == 0;
+  const bool pn = act && nb != 0 && atomic_or64(a.nmask[s.round & 1] + p.v, nb) == 0 && near_admit(a, s, p.v);
   const bool pf = act && fb != 0 && atomic_or64(a.fmask[s.phase & 1] + p.v, fb) == 0;
   wave_push(pn, p.v, a.q[s.near_q], &a.ctrl->qcount[s.near_q], a.qcap, &a.ctrl->abort);
   wave_push(pf, p.v, a.q[s.far_q], &a.ctrl->qcount[s.far_q], a.qcap, &a.ctrl->abort);
   p.n = 0;
 }
 
-// Relaxes the 16 edges with destinations vv and weights ww leaving an item
-// whose distance for this lane's source is du (INF: no candidate); the first
-// `skip` of them carry no candidate (relaxed already).  The 16 row loads are
-// issued back to back, then the atomicMins of the lanes whose candidate beats
-// the loaded distance.
-__device__ __forceinline__ void relax_group(const MArgs &a, LaneSched &ls, uint32_t du, const U32x16 &vv,
-                                            const U32x16 &ww, uint32_t skip, Pending &p, uint64_t &improved) {
-  const uint32_t lane = lane_id();
-  // lanes >= S have no source and du = INF: they load the row's last word
-  // and never issue an atomic
-  const uint32_t col = min(lane, a.S - 1);
-  uint32_t nd[kGroup], cur[kGroup];
+// One window of kGroup edges in flight: candidates nd, then the loaded
+// distances, then the atomics' results; dst holds destination k in lane k.
+struct Window {
+  uint32_t nd[kGroup];
+  uint32_t val[kGroup];
+  uint32_t dst;
+};
+
+// Issues the row loads of a window of edges (destinations vv, weights ww)
+// leaving an item whose distance for this lane's source is du (INF: no
+// candidate); the first `skip` edges carry no candidate (relaxed already).
+__device__ __forceinline__ void window_load(const MArgs &a, uint32_t du, const U32xG &vv, const U32xG &ww,
+                                            uint32_t skip, uint32_t col, Window &w) {
 #pragma unroll
   for (int k = 0; k < kGroup; ++k) {
     // du + w never wraps (du <= INF, w <= INT_MAX); INF + w and a skipped
     // slot's du + 2^31 both exceed every stored distance
-    nd[k] = du + ((uint32_t)k < skip ? 0x80000000u : ww[k]);
+    w.nd[k] = du + ((uint32_t)k < skip ? 0x80000000u : ww[k]);
   }
 #pragma unroll
-  for (int k = 0; k < kGroup; ++k) cur[k] = ld_coh(a.D + (size_t)vv[k] * a.S + col);
+  for (int k = 0; k < kGroup; ++k) w.val[k] = ld_coh(a.D + (size_t)vv[k] * a.S + col);
+}
+
+// Issues the atomicMins of the lanes whose candidate beats the loaded distance.
+__device__ __forceinline__ void window_atomics(const MArgs &a, const U32xG &vv, uint32_t col, Window &w) {
 #pragma unroll
   for (int k = 0; k < kGroup; ++k)
-    cur[k] = nd[k] < cur[k] ? atomic_min(a.D + (size_t)vv[k] * a.S + col, nd[k]) : 0u;
-  // Results are taken only here, after every atomic of the group has been
-  // issued (left to itself the compiler tests each result inside its branch,
-  // waiting for the whole memory queue after every atomic).
+    w.val[k] = w.nd[k] < w.val[k] ? atomic_min(a.D + (size_t)vv[k] * a.S + col, w.nd[k]) : 0u;
+}
+
+// Takes a window's atomic results (issued earlier, so the wait overlaps the
+// loads issued since) and records its improved edges.
+__device__ __forceinline__ void window_results(const MArgs &a, LaneSched &ls, Window &w, Pending &p,
+                                               uint64_t &improved) {
+  const uint32_t lane = lane_id();
+  // Results are taken only here (left to itself the compiler tests each
+  // result inside its atomic's branch, waiting for the whole memory queue).
 #pragma unroll
-  for (int k = 0; k < kGroup; ++k) asm volatile("" : "+v"(cur[k]));
+  for (int k = 0; k < kGroup; ++k) asm volatile("" : "+v"(w.val[k]));
 #pragma unroll
   for (int k = 0; k < kGroup; ++k) {
-    const bool imp = nd[k] < cur[k];
+    const bool imp = w.nd[k] < w.val[k];
     const uint64_t im = __ballot(imp);
     if (im == 0) continue;
-    const bool nearl = imp && nd[k] <= ls.thr32;
+    const uint32_t v = readlane(w.dst, k);
+    const bool nearl = imp && w.nd[k] <= ls.thr32;
     const uint64_t nb = __ballot(nearl), fb = im & ~nb;
-    if (fb && imp && !nearl) ls.fmin = min(ls.fmin, nd[k]);
+    if (fb && imp && !nearl) ls.fmin = min(ls.fmin, w.nd[k]);
     improved += 1;
     if (lane == p.n) {
-      p.v = vv[k];
+      p.v = v;
       p.nb_lo = (uint32_t)nb;
       p.nb_hi = (uint32_t)(nb >> 32);
       p.fb_lo = (uint32_t)fb;
@@ -251,21 +282,45 @@ __device__ __forceinline__ void relax_group(const MArgs &a, LaneSched &ls, uint3
   }
 }
 
-// Relaxes edges [lo, hi) of one item with at least 16 edges (distance du for
-// this lane's source), 16 at a time; the last window is shifted back to end
+// Relaxes edges [lo, hi) of one item with at least kGroup edges (distance du
+// for this lane's source), kGroup at a time.  The last window is shifted back to end
 // at hi, so every window lies inside the item's edges, and its slots that
-// the previous window covered are skipped.
+// the previous window covered are skipped.  Software-pipelined: while a
+// window's row loads are in flight the next window's edges are fetched and
+// the previous window's atomic results are taken, so a window costs about one
+// memory round trip instead of three.
 __device__ __forceinline__ void relax_edges(const MArgs &a, const Sched &s, LaneSched &ls, uint32_t du, uint32_t lo,
                                             uint32_t hi, Pending &p, uint64_t &improved) {
+  const uint32_t lane = lane_id();
+  // lanes >= S have no source and du = INF: they load the row's last word
+  // and never issue an atomic
+  const uint32_t col = min(lane, a.S - 1);
   // the bounds are wave-uniform; say so, so that they stay in scalar registers
   lo = __builtin_amdgcn_readfirstlane(lo);
   hi = __builtin_amdgcn_readfirstlane(hi);
-  for (uint32_t e = lo; e < hi; e += kGroup) {
+  uint32_t e = lo, w0 = min(lo, hi - kGroup);
+  U32xG vv = scalar_window(a.edge_dst, w0), ww = scalar_window(a.edge_data, w0);
+  Window cur, prev;
+  bool have_prev = false;
+  for (;;) {
+    window_load(a, du, vv, ww, e - w0, col, cur);
+    cur.dst = lane < kGroup ? a.edge_dst[w0 + lane] : 0u;
+    const uint32_t en = e + kGroup;
+    const bool more = en < hi;
+    const uint32_t wn = more ? min(en, hi - kGroup) : w0;
+    const U32xG vvn = scalar_window(a.edge_dst, wn), wwn = scalar_window(a.edge_data, wn);
+    if (have_prev) window_results(a, ls, prev, p, improved);
     if (p.n > (uint32_t)(kWave - kGroup)) flush_pending(a, s, p);
-    const uint32_t w0 = min(e, hi - kGroup);
-    const U32x16 vv = scalar_x16(a.edge_dst, w0), ww = scalar_x16(a.edge_data, w0);
-    relax_group(a, ls, du, vv, ww, e - w0, p, improved);
+    window_atomics(a, vv, col, cur);
+    prev = cur;
+    have_prev = true;
+    if (!more) break;
+    vv = vvn;
+    ww = wwn;
+    e = en;
+    w0 = wn;
   }
+  window_results(a, ls, prev, p, improved);
 }
 
 // Queues heavy vertex u (edges [rs, re)) as chunks of heavy_deg edges, each
@@ -304,6 +359,11 @@ __device__ __forceinline__ void process_items(const MArgs &a, const Sched &s, La
   uint32_t rs = 0, re = 0;
   if (u < a.nnodes) {
     m = atomic_exch64(mp + u, 0ull);
+#if LSG_STEAL
+    // The bits set for the next round so far are expanded now, with the
+    // distances that set them; u stays queued for the next round.
+    m |= atomic_exch64(a.nmask[s.round & 1] + u, 0ull);
+#endif
     rs = a.row_start[u];
     re = a.row_start[u + 1];
   }
@@ -608,6 +668,7 @@ __global__ void multi_init(MArgs a, MultiCounters *tot) {
     a.nmask[1][v] = 0;
     a.fmask[0][v] = 0;
     a.fmask[1][v] = 0;
+    a.qstamp[v] = 0;
   }
   if (tid == 0) *tot = MultiCounters{0, 0, 0};
 }
@@ -622,6 +683,7 @@ __global__ void multi_seed(MArgs a) {
   if (act) {
     a.D[(size_t)src * a.S + lane] = 0u;
     push = atomic_or64(a.nmask[0] + src, 1ull << lane) == 0;
+    a.qstamp[src] = 1u;  // in round 1's queue
   }
   uint64_t pm = __ballot(push);
   if (push) a.q[0][lanes_below(pm)] = src;
@@ -690,14 +752,16 @@ static int run_multi(Workspace *w, const lsg_graph *g, const uint32_t *srcs_host
   int rc;
   int bs = o.block_size ? o.block_size : 512;
   if (bs != 256 && bs != 512 && bs != 1024) return fail(LSG_ERR_ARG, "block_size must be 256, 512 or 1024");
-  uint32_t heavy_deg = o.heavy_deg > 0 ? (uint32_t)o.heavy_deg : 2048u;
+  // default chunk size, measured on RMAT-22 x 64 sources: 512 30.0 ms, 256
+  // 34.3, 1024 30.6, 128 63
+  uint32_t heavy_deg = o.heavy_deg > 0 ? (uint32_t)o.heavy_deg : 512u;
   if (heavy_deg < kLongDeg) return fail(LSG_ERR_ARG, "heavy_deg must be >= %u for batched runs", kLongDeg);
   uint32_t V = g->nnodes;
   uint64_t hcap64 = 2ull * g->nedges / heavy_deg + 64;
   uint32_t hcap = (uint32_t)std::min<uint64_t>(hcap64, UINT32_MAX / 4);
   void *qbuf, *hbuf, *mbuf, *sbuf;
   if ((rc = ws_buffer(w, kBufQueues, 3ull * V * 4, &qbuf)) || (rc = ws_buffer(w, kBufHeavy, 2ull * hcap * 16, &hbuf)) ||
-      (rc = ws_buffer(w, kBufMasks, 4ull * V * 8 + 256, &mbuf)) || (rc = ws_buffer(w, kBufSeeds, 2048, &sbuf)))
+      (rc = ws_buffer(w, kBufMasks, 4ull * V * 8 + 4ull * V + 256, &mbuf)) || (rc = ws_buffer(w, kBufSeeds, 2048, &sbuf)))
     return rc;
   MArgs a;
   memset(&a, 0, sizeof a);
@@ -710,6 +774,7 @@ static int run_multi(Workspace *w, const lsg_graph *g, const uint32_t *srcs_host
   a.nmask[1] = masks + V;
   a.fmask[0] = masks + 2ull * V;
   a.fmask[1] = masks + 3ull * V;
+  a.qstamp = (uint32_t *)(masks + 4ull * V);
   for (int i = 0; i < 3; ++i) a.q[i] = (uint32_t *)qbuf + (size_t)i * V;
   a.hq[0] = (uint64_t *)hbuf;
   a.hq[1] = (uint64_t *)hbuf + 2ull * hcap;

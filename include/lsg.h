@@ -85,7 +85,7 @@ typedef struct lsg_opts {
   int32_t grid_blocks; /* persistent grid; 0 = every block the device can keep resident */
   int32_t thread_deg;  /* TWB: degrees below this are expanded by one lane (default 8) */
   int32_t heavy_deg;   /* degrees >= this are split into chunks of heavy_deg edges, expanded by whole
-                          blocks in the next round (default 512 single-source, 2048 batched) */
+                          blocks in the next round (default 512; batched runs need >= 8) */
   int32_t sched;       /* 0 = fixed delta; 1 = the learned scheduler picks each phase's delta width
                           and mode (expansion mode for SSSP, far mode for batched runs) from the
                           model loaded with lsg_sched_set_model */
