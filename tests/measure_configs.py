@@ -203,8 +203,8 @@ def c5(repeats: int) -> dict:
     D = DeviceArray(V * 32 * 4)
     buf = DeviceArray(V * 4)
     learned = have_model("batched")
-    bo = Options(delta=32, sched=1, heavy_deg=256, block_size=256) if learned else \
-        Options(delta=1 << 30, far_mode=2, heavy_deg=256, block_size=256)
+    bo = Options(delta=32, sched=1, heavy_deg=512, block_size=512) if learned else \
+        Options(delta=1 << 30, far_mode=2, heavy_deg=512, block_size=512)
     sssp_multi_into(dev, srcs, bo, D)
     Dh = D.to_numpy(np.int32, (V, 32))
     er = vr = 0
