@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import oracle
-from galoisgpu_amd import CSRGraph, Options
+from galoisgpu_amd import CSRGraph, LsgError, Options
 from galoisgpu_amd import sched as lsched
 from galoisgpu_amd.batch import last_features, sssp_batch, sssp_multi
 
@@ -40,6 +40,51 @@ def test_multi_small_golden(small_graphs):
                 d, _ = sssp_multi(dev, srcs, Options(delta=delta, far_mode=far))
                 assert np.array_equal(d, ref), (name, delta, far)
         dev.close()
+
+
+def degree_boundary_graph():
+    """Vertices whose out-degrees sit on both sides of every boundary of the
+    batched kernel's expansion paths: the 8-edge relax window (7, 8, 9, 15,
+    16, 17), and chunk sizes heavy_deg = 8, 16, 17 and 64 (a chunk of one edge,
+    a chunk shorter than a window).  The last vertex has 37 edges, so its last
+    window ends at the last edge of the array.  Weights include zeros."""
+    rng = np.random.default_rng(11)
+    V = 3000
+    deg = rng.integers(0, 6, V)
+    deg[:50] = np.maximum(deg[:50], 2)
+    special = [1, 7, 8, 9, 15, 16, 17, 18, 31, 33, 63, 64, 65, 127, 128, 129, 136, 255, 257, 1025]
+    hubs = rng.choice(np.arange(1, V - 1), len(special), replace=False)
+    deg[hubs] = special
+    deg[V - 1] = 37
+    row = np.zeros(V + 1, dtype=np.int64)
+    row[1:] = np.cumsum(deg)
+    E = int(row[-1])
+    col = rng.integers(0, V, E)
+    # every hub is reached from the low vertices the sources are drawn from
+    for k, h in enumerate(hubs):
+        if deg[k % 50] > 0:
+            col[row[k % 50]] = h
+    w = rng.integers(0, 100, E)
+    w[rng.random(E) < 0.6] += 1
+    return CSRGraph.from_arrays(row, col, w), hubs
+
+
+def test_multi_degree_boundaries():
+    g, hubs = degree_boundary_graph()
+    srcs = np.concatenate([np.arange(0, 50, 3), hubs[:8]]).astype(np.uint32)
+    srcs = np.resize(srcs, 64)
+    ref = oracle_cols(g, srcs)
+    dev = g.copy_to_gpu()
+    for hd in (8, 16, 17, 64, 0):
+        for S in (1, 7, 64):
+            for far, delta in ((0, 1 << 30), (2, 40), (0, 13)):
+                o = Options(delta=delta, far_mode=far, heavy_deg=hd, block_size=256)
+                say("degree boundaries", hd, S, far, delta)
+                d, _ = sssp_multi(dev, srcs[:S], o)
+                assert np.array_equal(d, ref[:, :S]), (hd, S, far, delta)
+    with pytest.raises(LsgError, match="heavy_deg"):
+        sssp_multi(dev, srcs[:4], Options(delta=32, heavy_deg=7))
+    dev.close()
 
 
 @pytest.fixture(scope="module")
