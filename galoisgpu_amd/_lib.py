@@ -159,6 +159,8 @@ SIGNATURES = {
     "lsg_sssp_dist": (c_int, [POINTER(LsgGraph), POINTER(LsgOpts), c_void_p, POINTER(LsgStats)]),
     "lsg_bfs_dist": (c_int, [POINTER(LsgGraph), POINTER(LsgOpts), c_void_p, POINTER(LsgStats)]),
     "lsg_last_trace": (c_int, [c_int, POINTER(LsgRoundTrace), c_uint32, POINTER(c_uint32), POINTER(c_double)]),
+    "lsg_graph_transpose": (c_int, [POINTER(LsgGraph), POINTER(LsgGraph)]),
+    "lsg_bfs_do": (c_int, [POINTER(LsgGraph), POINTER(LsgGraph), c_uint32, c_int, c_void_p, POINTER(LsgStats)]),
     "lsg_output": (c_int, [POINTER(LsgGraph), c_char_p]),
     "lsg_output_dist": (c_int, [c_void_p, c_uint32, c_char_p]),
     "lsg_sssp_multi": (c_int, [POINTER(LsgGraph), c_void_p, c_uint32, POINTER(LsgOpts), c_void_p,

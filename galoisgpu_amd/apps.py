@@ -92,6 +92,18 @@ def bfs_into(dev: DeviceGraph, level_ptr: int, opts: Options) -> dict:
     return st.as_dict()
 
 
+BFS_AUTO, BFS_TOP_DOWN, BFS_BOTTOM_UP = 0, 1, 2
+
+
+def bfs_do_into(dev: DeviceGraph, rev: DeviceGraph, level_ptr: int, start_node: int = 0,
+                direction: int = BFS_AUTO) -> dict:
+    """Direction-optimising BFS (lsg_bfs_do); `rev` is dev.transpose()."""
+    st = LsgStats()
+    check(lib().lsg_bfs_do(ctypes.byref(dev._g), ctypes.byref(rev._g), start_node, direction,
+                           ctypes.c_void_p(level_ptr), ctypes.byref(st)), "bfs_do")
+    return st.as_dict()
+
+
 def _run(graph: CSRGraph, opts: Options, device: int, fn) -> Tuple[np.ndarray, dict]:
     dev = graph.copy_to_gpu(device)
     try:

@@ -166,6 +166,13 @@ class DeviceGraph:
     def copy_to_cpu(self, host: CSRGraph) -> None:
         check(lib().lsg_graph_copy_to_cpu(ctypes.byref(self._g), ctypes.byref(host._g)), "copy_to_cpu")
 
+    def transpose(self) -> "DeviceGraph":
+        """Reverse CSR of this graph, built on its device (lsg_graph_transpose)."""
+        t = DeviceGraph()
+        check(lib().lsg_graph_transpose(ctypes.byref(self._g), ctypes.byref(t._g)), "transpose")
+        t._owned = True
+        return t
+
     def close(self) -> None:
         if self._owned:
             lib().lsg_graph_dealloc(ctypes.byref(self._g))

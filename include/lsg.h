@@ -153,6 +153,11 @@ int lsg_graph_copy_to_gpu(const lsg_graph *host, lsg_graph *dev, int device);
 int lsg_graph_copy_to_cpu(const lsg_graph *dev, lsg_graph *host);
 /* CSRGraph::dealloc (rt/src/csr_graph.cu:243-248). */
 void lsg_graph_dealloc(lsg_graph *g);
+/* Reverse CSR of a device graph, built on its device: gt's row v lists the
+ * sources of v's in-edges (in no particular order) with their weights.  gt
+ * is allocated here and freed with lsg_graph_dealloc.  No reference
+ * counterpart; bottom-up BFS steps read it. */
+int lsg_graph_transpose(const lsg_graph *dev, lsg_graph *gt);
 /* Wraps caller-owned device arrays (for example torch tensors) as a device
  * graph without copying.  node_data may be NULL when only *_dist entry points
  * are used. Nothing is freed by lsg_graph_dealloc for such graphs. */
@@ -187,6 +192,15 @@ int lsg_bfs_gg_main(lsg_graph *hg, lsg_graph *gg, const lsg_opts *opts, lsg_stat
  * several result arrays resident). */
 int lsg_sssp_dist(const lsg_graph *gg, const lsg_opts *opts, int32_t *dist_dev, lsg_stats *stats);
 int lsg_bfs_dist(const lsg_graph *gg, const lsg_opts *opts, int32_t *level_dev, lsg_stats *stats);
+/* Direction-optimising BFS from `src` (no reference counterpart as an
+ * algorithm; the levels are apps/bfs's, as lsg_bfs_dist writes them).  `gt`
+ * is the reverse of `g` (lsg_graph_transpose).  direction: 0 = per level by
+ * frontier size (top-down, bottom-up once the frontier's edges exceed 1/14 of
+ * the unexplored edges, top-down again below V/24 frontier vertices), 1 =
+ * top-down only, 2 = bottom-up only.  level_dev is a device array of nnodes.
+ * stats: iterations = levels, near_iters = bottom-up levels. */
+int lsg_bfs_do(const lsg_graph *g, const lsg_graph *gt, uint32_t src, int direction, int32_t *level_dev,
+               lsg_stats *stats);
 /* Copies the round records of the last traced call on `device` into out[cap];
  * *n gets the number of records, *tick_hz the device wall-clock rate. */
 int lsg_last_trace(int device, lsg_round_trace *out, uint32_t cap, uint32_t *n, double *tick_hz);

@@ -1,0 +1,129 @@
+"""GPU parity of the direction-optimising BFS and of the reverse CSR it reads.
+
+Levels must equal the oracle's level-synchronous BFS (apps/bfs/kernel.cu:
+312-334) bit for bit under every direction policy: automatic, top-down only
+and bottom-up only.  The reverse CSR is checked against a numpy transpose row
+by row (as multisets: its rows come out of atomics, unordered).
+"""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from galoisgpu_amd import CSRGraph, LsgError, lib
+from galoisgpu_amd.apps import BFS_AUTO, BFS_BOTTOM_UP, BFS_TOP_DOWN, bfs_do_into
+from galoisgpu_amd.batch import DeviceArray
+
+pytestmark = pytest.mark.gpu
+D2H = 1
+
+
+def say(*a):
+    print("[gpu-test]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def fetch(ptr, n, device=0):
+    out = np.empty(n, dtype=np.uint32)
+    if n:
+        assert lib().lsg_dev_copy(device, out.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(ptr), out.nbytes,
+                                  D2H) == 0
+    return out
+
+
+def check_levels(g, srcs, dirs=(BFS_AUTO, BFS_TOP_DOWN, BFS_BOTTOM_UP)):
+    dev = g.copy_to_gpu()
+    rev = dev.transpose()
+    lv = DeviceArray(max(1, g.nnodes) * 4)
+    try:
+        for s in srcs:
+            ref, _ = oracle.bfs(g.row_start, g.edge_dst, int(s))
+            for d in dirs:
+                st = bfs_do_into(dev, rev, lv.ptr, int(s), d)
+                got = lv.to_numpy(np.int32, g.nnodes)
+                assert np.array_equal(got, ref), (s, d)
+                if d == BFS_BOTTOM_UP:
+                    assert st["near_iters"] == st["iterations"] + 1  # every launch bottom-up
+    finally:
+        lv.close()
+        rev.close()
+        dev.close()
+
+
+def test_transpose_matches_numpy():
+    g = CSRGraph.rmat(12, 8, seed=4)
+    dev = g.copy_to_gpu()
+    rev = dev.transpose()
+    V, E = g.nnodes, g.nedges
+    row = fetch(ctypes.cast(rev._g.row_start, ctypes.c_void_p).value, V + 1)
+    src = fetch(ctypes.cast(rev._g.edge_dst, ctypes.c_void_p).value, E)
+    w = fetch(ctypes.cast(rev._g.edge_data, ctypes.c_void_p).value, E)
+    r = g.row_start.astype(np.int64)
+    col = g.edge_dst.astype(np.int64)
+    sources = np.repeat(np.arange(V), np.diff(r))
+    indeg = np.bincount(col, minlength=V)
+    assert np.array_equal(row, np.concatenate([[0], np.cumsum(indeg)]).astype(np.uint32))
+    fwd = sorted(zip(col.tolist(), sources.tolist(), g.edge_data.astype(np.int64).tolist()))
+    back = sorted((v, int(src[e]), int(w[e])) for v in range(V) for e in range(row[v], row[v + 1]))
+    assert fwd == back
+    rev.close()
+    dev.close()
+
+
+def test_bfs_do_golden(small_graphs):
+    for name, gg in small_graphs.items():
+        g = CSRGraph.from_arrays(gg["row"], gg["col"], gg["w"])
+        say("bfs_do golden", name)
+        check_levels(g, [gg["src"], 0, g.nnodes - 1])
+
+
+def test_bfs_do_grid_closed_form():
+    # unit-weight grid: the level of (r, c) from (0, 0) is r + c
+    g = CSRGraph.grid(40, 70, 1, 1, seed=1)
+    dev = g.copy_to_gpu()
+    rev = dev.transpose()
+    lv = DeviceArray(g.nnodes * 4)
+    for d in (BFS_AUTO, BFS_TOP_DOWN, BFS_BOTTOM_UP):
+        bfs_do_into(dev, rev, lv.ptr, 0, d)
+        got = lv.to_numpy(np.int32, g.nnodes).reshape(40, 70)
+        assert np.array_equal(got, np.add.outer(np.arange(40), np.arange(70))), d
+    lv.close()
+    rev.close()
+    dev.close()
+
+
+@pytest.mark.parametrize("scale,ef", [(10, 4), (14, 16), (16, 8)])
+def test_bfs_do_rmat(scale, ef):
+    g = CSRGraph.rmat(scale, ef, seed=scale)
+    srcs = [g.max_degree_vertex()] + [int(x) for x in np.random.default_rng(scale).integers(0, g.nnodes, 3)]
+    say("bfs_do rmat", scale, ef)
+    check_levels(g, srcs)
+
+
+def test_bfs_do_roadlike():
+    g = CSRGraph.roadlike(96, 96, 0.1, 0.001, 1, 100, seed=2)
+    say("bfs_do roadlike")
+    check_levels(g, [0, 4000])
+
+
+def test_bfs_do_edge_cases():
+    # a single vertex, a sink source, and an isolated vertex beside a component
+    one = CSRGraph.from_arrays(np.array([0, 0]), np.array([], dtype=np.uint32))
+    check_levels(one, [0])
+    g = CSRGraph.from_arrays(np.array([0, 1, 2, 2, 2]), np.array([1, 2]))
+    check_levels(g, [0, 2, 3])
+    dev = g.copy_to_gpu()
+    rev = dev.transpose()
+    lv = DeviceArray(g.nnodes * 4)
+    with pytest.raises(LsgError):
+        bfs_do_into(dev, rev, lv.ptr, g.nnodes, BFS_AUTO)
+    with pytest.raises(LsgError):
+        bfs_do_into(dev, rev, lv.ptr, 0, 3)
+    with pytest.raises(LsgError, match="reverse graph"):
+        other = CSRGraph.rmat(4, 2, seed=1).copy_to_gpu()
+        bfs_do_into(dev, other, lv.ptr, 0, BFS_AUTO)
+    lv.close()
+    rev.close()
+    dev.close()
