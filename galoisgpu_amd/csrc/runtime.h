@@ -39,6 +39,7 @@ struct Workspace {
   int cus = 0;
   double wall_khz = 0;  // device wall clock rate (wall_clock64 ticks per ms)
   Ctrl *ctrl = nullptr;
+  uint32_t *pinned = nullptr;  // 256 bytes of pinned host memory for small readbacks
   void *buf[kBufCount] = {};
   size_t buf_bytes[kBufCount] = {};
   struct GridEntry {

@@ -30,6 +30,7 @@ int get_workspace(int device, Workspace **out) {
   HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device));
   w->wall_khz = khz > 0 ? khz : 100000.0;
   HIPCHK(hipMalloc(&w->ctrl, sizeof(Ctrl)));
+  HIPCHK(hipHostMalloc((void **)&w->pinned, 256, hipHostMallocDefault));
   g_ws.push_back(w);
   *out = w;
   return LSG_OK;

@@ -28,13 +28,20 @@ static const char *prog_opts = "ls:d:";
 static const char *prog_usage = "[-l] [-d delta] [-s startNode]";
 static const char *GGC_OPTIONS = "lsg-mi355x sssp: persistent near/far, wave64, gfx950";
 #elif defined(LSG_APP_BFS)
-static const char *prog_opts = "ls:";
-static const char *prog_usage = "[-l] [-s startNode]";
+static const char *prog_opts = "lDs:";
+static const char *prog_usage = "[-l] [-D] [-s startNode]";
 static const char *GGC_OPTIONS = "lsg-mi355x bfs: persistent level-synchronous, wave64, gfx950";
 #else
 #error "define LSG_APP_SSSP or LSG_APP_BFS"
 #endif
+#if defined(LSG_APP_BFS)
+static const char *prog_args_usage =
+    "-l: enable thread block load balancer (by default false)\n"
+    " -D: direction-optimising BFS over a reverse CSR built on the GPU before the timed region";
+#else
 static const char *prog_args_usage = "-l: enable thread block load balancer (by default false)";
+#endif
+static int DIRECTION_OPT = 0;  // bfs -D
 
 static int QUIET = 0;
 static char *INPUT = nullptr, *OUTPUT = nullptr;
@@ -60,6 +67,7 @@ static void process_prog_opt(char c, char *optarg) {
     }
   }
   if (c == 'l') OPTS.enable_lb = 1;
+  if (c == 'D') DIRECTION_OPT = 1;
   if (c == 's') {
     OPTS.start_node = atoi(optarg);
     if (OPTS.start_node < 0) {
@@ -115,11 +123,24 @@ static int load_graph_and_run_kernel(char *graph_file) {
 #if defined(LSG_APP_SSSP)
   printf("delta: %d\n", OPTS.delta);  // apps/sssp/kernel.cu:473
 #endif
+#if defined(LSG_APP_BFS)
+  lsg_graph gt;
+  memset(&gt, 0, sizeof gt);
+  if (DIRECTION_OPT) {
+    uint64_t p0 = now_ns();
+    rc = lsg_graph_transpose(&gg, &gt);
+    if (rc) die("transpose", rc);
+    fprintf(stderr, "transpose: %llu us\n", (unsigned long long)((now_ns() - p0) / 1000ull));
+  }
+#endif
   uint64_t t0 = now_ns();
 #if defined(LSG_APP_SSSP)
   rc = lsg_sssp_gg_main(&g, &gg, &OPTS, &st);
 #else
-  rc = lsg_bfs_gg_main(&g, &gg, &OPTS, &st);
+  if (DIRECTION_OPT)  // lsg_bfs_do rejects a source out of range
+    rc = lsg_bfs_do(&gg, &gt, (uint32_t)OPTS.start_node, 0, gg.node_data, &st);
+  else
+    rc = lsg_bfs_gg_main(&g, &gg, &OPTS, &st);
 #endif
   if (rc == LSG_OK) rc = lsg_device_sync(CUDA_DEVICE);
   uint64_t t1 = now_ns();
@@ -137,6 +158,9 @@ static int load_graph_and_run_kernel(char *graph_file) {
     rc = lsg_output(&g, OUTPUT);
     if (rc) die("output", rc);
   }
+#if defined(LSG_APP_BFS)
+  if (DIRECTION_OPT) lsg_graph_dealloc(&gt);
+#endif
   lsg_graph_dealloc(&gg);
   lsg_graph_dealloc(&g);
   return 0;

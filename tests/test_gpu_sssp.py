@@ -245,6 +245,12 @@ def test_cli_output_matches_oracle_text(tmp_path):
     lv, _ = oracle.bfs(g.row_start, g.edge_dst, 5)
     want = "".join(("%d INF\n" % i) if x == INF else ("%d %d\n" % (i, x)) for i, x in enumerate(lv))
     assert open(out2).read() == want
+    say("cli bfs -D")
+    out3 = str(tmp_path / "bfs_do.txt")
+    r = subprocess.run([exe, "-D", "-s", "5", "-o", out3, gr], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "transpose:" in r.stderr and "Total time:" in r.stderr
+    assert open(out3).read() == want
 
 
 def test_bmk_harness_runs_the_cli(tmp_path):
