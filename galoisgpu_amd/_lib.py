@@ -160,6 +160,9 @@ SIGNATURES = {
     "lsg_bfs_dist": (c_int, [POINTER(LsgGraph), POINTER(LsgOpts), c_void_p, POINTER(LsgStats)]),
     "lsg_last_trace": (c_int, [c_int, POINTER(LsgRoundTrace), c_uint32, POINTER(c_uint32), POINTER(c_double)]),
     "lsg_graph_transpose": (c_int, [POINTER(LsgGraph), POINTER(LsgGraph)]),
+    "lsg_cc": (c_int, [POINTER(LsgGraph), c_void_p, POINTER(c_uint32), POINTER(LsgStats)]),
+    "lsg_cc_gg_main": (c_int, [POINTER(LsgGraph), POINTER(LsgGraph), POINTER(LsgStats)]),
+    "lsg_output_cc": (c_int, [POINTER(LsgGraph), c_char_p]),
     "lsg_bfs_do": (c_int, [POINTER(LsgGraph), POINTER(LsgGraph), c_uint32, c_int, c_void_p, POINTER(LsgStats)]),
     "lsg_output": (c_int, [POINTER(LsgGraph), c_char_p]),
     "lsg_output_dist": (c_int, [c_void_p, c_uint32, c_char_p]),

@@ -92,6 +92,24 @@ def bfs_into(dev: DeviceGraph, level_ptr: int, opts: Options) -> dict:
     return st.as_dict()
 
 
+def cc_into(dev: DeviceGraph, label_ptr: int) -> dict:
+    """Connected components (lsg_cc): label[v] = smallest id of v's component
+    (edge directions ignored); stats['items'] is the number of components."""
+    st = LsgStats()
+    n = ctypes.c_uint32()
+    check(lib().lsg_cc(ctypes.byref(dev._g), ctypes.c_void_p(label_ptr), ctypes.byref(n), ctypes.byref(st)), "cc")
+    return st.as_dict()
+
+
+def cc(graph: CSRGraph, device: int = 0) -> Tuple[np.ndarray, dict]:
+    """apps/cc gg_main on a host graph: (labels, stats)."""
+    def run(g, dev, _opts):
+        st = LsgStats()
+        check(lib().lsg_cc_gg_main(ctypes.byref(g._g), ctypes.byref(dev._g), ctypes.byref(st)), "cc")
+        return st.as_dict()
+    return _run(graph, Options(), device, run)
+
+
 BFS_AUTO, BFS_TOP_DOWN, BFS_BOTTOM_UP = 0, 1, 2
 
 

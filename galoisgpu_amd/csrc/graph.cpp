@@ -355,6 +355,31 @@ int lsg_output(const lsg_graph *g, const char *path) {
   return lsg_output_dist(g->node_data, g->nnodes, path);
 }
 
+int lsg_output_cc(const lsg_graph *g, const char *path) {
+  if (!g) return fail(LSG_ERR_ARG, "output: NULL graph");
+  if (g->device_graph) return fail(LSG_ERR_ARG, "output: expected a host graph (copy_to_cpu first)");
+  if (!path) return LSG_OK;  // apps/cc/support.cu:20-21
+  if (!g->node_data) return fail(LSG_ERR_ARG, "output: NULL labels");
+  bool own = strcmp(path, "-") != 0;
+  FILE *f = own ? fopen(path, "w") : stdout;  // support.cu:23-28
+  if (!f) return fail(LSG_ERR_IO, "unable to open %s for writing: %s", path, strerror(errno));
+  bool ok = true;
+  // support.cu:30-35: the node's label, then each out-edge with its weight and
+  // the destination's label
+  for (uint32_t i = 0; i < g->nnodes && ok; ++i) {
+    ok = fprintf(f, "%d %d\n", (int)i, g->node_data[i]) > 0;
+    for (uint32_t j = g->row_start[i]; j < g->row_start[i + 1] && ok; ++j) {
+      const uint32_t dst = g->edge_dst[j];
+      ok = fprintf(f, "\te %d: %d %d %d\n", (int)j, (int)dst, g->edge_data ? g->edge_data[j] : 0,
+                   g->node_data[dst]) > 0;
+    }
+  }
+  if (own) ok = (fclose(f) == 0) && ok;
+  else fflush(f);
+  if (!ok) return fail(LSG_ERR_IO, "write to %s failed", path);
+  return LSG_OK;
+}
+
 uint32_t lsg_max_degree_vertex(const lsg_graph *g) {
   if (!g || !g->row_start || g->device_graph) return 0;
   uint32_t best = 0, bd = 0;

@@ -31,10 +31,16 @@ static const char *GGC_OPTIONS = "lsg-mi355x sssp: persistent near/far, wave64, 
 static const char *prog_opts = "lDs:";
 static const char *prog_usage = "[-l] [-D] [-s startNode]";
 static const char *GGC_OPTIONS = "lsg-mi355x bfs: persistent level-synchronous, wave64, gfx950";
+#elif defined(LSG_APP_CC)
+static const char *prog_opts = "";
+static const char *prog_usage = "";
+static const char *GGC_OPTIONS = "lsg-mi355x cc: hooking + pointer jumping, wave64, gfx950";
 #else
-#error "define LSG_APP_SSSP or LSG_APP_BFS"
+#error "define LSG_APP_SSSP, LSG_APP_BFS or LSG_APP_CC"
 #endif
-#if defined(LSG_APP_BFS)
+#if defined(LSG_APP_CC)
+static const char *prog_args_usage = "";
+#elif defined(LSG_APP_BFS)
 static const char *prog_args_usage =
     "-l: enable thread block load balancer (by default false)\n"
     " -D: direction-optimising BFS over a reverse CSR built on the GPU before the timed region";
@@ -136,6 +142,8 @@ static int load_graph_and_run_kernel(char *graph_file) {
   uint64_t t0 = now_ns();
 #if defined(LSG_APP_SSSP)
   rc = lsg_sssp_gg_main(&g, &gg, &OPTS, &st);
+#elif defined(LSG_APP_CC)
+  rc = lsg_cc_gg_main(&g, &gg, &st);
 #else
   if (DIRECTION_OPT)  // lsg_bfs_do rejects a source out of range
     rc = lsg_bfs_do(&gg, &gt, (uint32_t)OPTS.start_node, 0, gg.node_data, &st);
@@ -147,6 +155,8 @@ static int load_graph_and_run_kernel(char *graph_file) {
   if (rc) die("gg_main", rc);
 #if defined(LSG_APP_SSSP)
   printf("iterations: %d\n", (int)st.iterations);  // apps/sssp/kernel.cu:479
+#elif defined(LSG_APP_CC)
+  printf("components: %d\n", (int)st.items);  // apps/cc/kernel.cu:719
 #endif
   printf("gg_main %llu %llu\n", (unsigned long long)t0, (unsigned long long)t1);  // Timer.h:77-79
   fprintf(stderr, "Total time: %llu ms\n", (unsigned long long)((t1 - t0) / 1000000ull));
@@ -155,7 +165,11 @@ static int load_graph_and_run_kernel(char *graph_file) {
   if (rc) die("copy_to_cpu", rc);
   if (!QUIET) {
     fflush(stdout);
+#if defined(LSG_APP_CC)
+    rc = lsg_output_cc(&g, OUTPUT);
+#else
     rc = lsg_output(&g, OUTPUT);
+#endif
     if (rc) die("output", rc);
   }
 #if defined(LSG_APP_BFS)

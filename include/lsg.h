@@ -192,6 +192,20 @@ int lsg_bfs_gg_main(lsg_graph *hg, lsg_graph *gg, const lsg_opts *opts, lsg_stat
  * several result arrays resident). */
 int lsg_sssp_dist(const lsg_graph *gg, const lsg_opts *opts, int32_t *dist_dev, lsg_stats *stats);
 int lsg_bfs_dist(const lsg_graph *gg, const lsg_opts *opts, int32_t *level_dev, lsg_stats *stats);
+/* Connected components (apps/cc/kernel.cu:630-723), edges taken without
+ * direction (weakly connected components).  label_dev[v] gets the smallest
+ * vertex id of v's component; *ncomponents (may be NULL) the number of
+ * components.  The reference's labels depend on its racing hooks; the
+ * partition is the same.  stats: iterations = hook passes, items =
+ * components. */
+int lsg_cc(const lsg_graph *g, int32_t *label_dev, uint32_t *ncomponents, lsg_stats *stats);
+/* gg_main of apps/cc: labels into gg->node_data ("components: %d" is
+ * stats->items, apps/cc/kernel.cu:719).  `hg` may be NULL. */
+int lsg_cc_gg_main(lsg_graph *hg, lsg_graph *gg, lsg_stats *stats);
+/* apps/cc/support.cu:17-36 output: "%d %d" per node (id, label), then per
+ * edge "\te %d: %d %d %d" (edge id, destination, weight, destination's label);
+ * file "-" is stdout. */
+int lsg_output_cc(const lsg_graph *host_g, const char *file);
 /* Direction-optimising BFS from `src` (no reference counterpart as an
  * algorithm; the levels are apps/bfs's, as lsg_bfs_dist writes them).  `gt`
  * is the reverse of `g` (lsg_graph_transpose).  direction: 0 = per level by

@@ -1,4 +1,4 @@
-"""CPU oracle for the SSSP/BFS path -- TEST INFRASTRUCTURE ONLY.
+"""CPU oracle for the SSSP/BFS/CC paths -- TEST INFRASTRUCTURE ONLY.
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
 import this package, and only as the checker.  It wraps oracle/liboracle.so
@@ -47,6 +47,8 @@ def lib() -> ctypes.CDLL:
     l.oracle_sssp_certificate.restype = i64
     l.oracle_reach.argtypes = [u32, P, P, P, P]
     l.oracle_reach.restype = None
+    l.oracle_cc.argtypes = [u32, P, P, P]
+    l.oracle_cc.restype = i64
     l.oracle_plant_bug.argtypes = [ctypes.c_int]
     l.oracle_plant_bug.restype = None
     _lib = l
@@ -114,6 +116,18 @@ def reach(row_start, dist) -> Tuple[int, int]:
     vr, er = ctypes.c_uint64(), ctypes.c_uint64()
     lib().oracle_reach(len(r) - 1, _p(r), _p(d), ctypes.byref(vr), ctypes.byref(er))
     return int(vr.value), int(er.value)
+
+
+def cc(row_start, edge_dst) -> Tuple[np.ndarray, int]:
+    """Weakly connected components (apps/cc): (label = smallest id of each
+    vertex's component, number of components)."""
+    r, c, _ = _arrs(row_start, edge_dst)
+    n = len(r) - 1
+    lab = np.empty(n, dtype=np.int32)
+    k = lib().oracle_cc(n, _p(r), _p(c), _p(lab))
+    if k < 0:
+        raise ValueError("oracle_cc failed")
+    return lab, int(k)
 
 
 def plant_bug(which: int) -> None:

@@ -16,6 +16,9 @@
  *                      :312-334 (level loop), :380-393 (LEVEL = 1)
  *   oracle_dijkstra    textbook binary-heap Dijkstra over the same int32
  *                      arithmetic (INF = INT_MAX, apps/sssp/kernel.cu:18)
+ *   oracle_cc          weakly connected components, labelled by the smallest
+ *                      id of each component (apps/cc/kernel.cu:630-723's
+ *                      partition; union-find)
  *   oracle_sssp_certificate  O(E) shortest-path certificate: d[src] = 0,
  *                      every edge out of a reached vertex is relaxed, every
  *                      reached vertex other than the source has a tight
@@ -278,4 +281,41 @@ void oracle_reach(uint32_t n, const uint32_t *row, const int32_t *dist, uint64_t
     }
   *vr = a;
   *er = b;
+}
+
+/* Connected components of the graph with edge directions ignored
+ * (apps/cc/kernel.cu:630-723 computes the same partition by hooking and
+ * pointer jumping).  label[v] = the smallest vertex id of v's component, a
+ * canonical form of the reference's schedule-dependent labels.  Union-find
+ * with union by smaller root and path halving.  Planted mistake 4 (tests
+ * only): the last edge of every row is skipped.  Returns the number of
+ * components. */
+static uint32_t uf_find(uint32_t *p, uint32_t x) {
+  while (p[x] != x) {
+    p[x] = p[p[x]];
+    x = p[x];
+  }
+  return x;
+}
+
+int64_t oracle_cc(uint32_t n, const uint32_t *row, const uint32_t *col, int32_t *label) {
+  uint32_t *p = (uint32_t *)malloc((size_t)n * sizeof(uint32_t));
+  if (!p) return -1;
+  for (uint32_t v = 0; v < n; ++v) p[v] = v;
+  for (uint32_t u = 0; u < n; ++u) {
+    const uint32_t end = row[u + 1] - (g_plant == 4 && row[u + 1] > row[u]);
+    for (uint32_t e = row[u]; e < end; ++e) {
+      uint32_t a = uf_find(p, u), b = uf_find(p, col[e]);
+      if (a == b) continue;
+      if (a < b) p[b] = a; /* the smaller id stays the root */
+      else p[a] = b;
+    }
+  }
+  int64_t comps = 0;
+  for (uint32_t v = 0; v < n; ++v) {
+    label[v] = (int32_t)uf_find(p, v);
+    comps += label[v] == (int32_t)v;
+  }
+  free(p);
+  return comps;
 }

@@ -14,6 +14,8 @@ oracle's serial near/far restatement timed beside it on the host.
   C4B 256 RMAT graphs, scales uniform in [10,16], ef 8, max-degree source each:
       one union launch vs the plain kernel graph by graph
   C5  RMAT-24 ef16, 32 sources: batched (+ learned) vs plain per source
+  CC  connected components (apps/cc) of the RMAT-22 and 2048x2048 road-like
+      graphs against the oracle's union-find
 
 The headline RMAT-22 x 64-source configuration is bench.py.  This file uses
 the oracle (test infrastructure) as the checker and CPU baseline, which is why
@@ -39,7 +41,7 @@ sys.path.insert(0, ROOT)
 import oracle  # noqa: E402
 from galoisgpu_amd import CSRGraph, Options  # noqa: E402
 from galoisgpu_amd import sched as lsched  # noqa: E402
-from galoisgpu_amd.apps import bfs_do_into, bfs_into, sssp_into  # noqa: E402
+from galoisgpu_amd.apps import bfs_do_into, bfs_into, cc_into, sssp_into  # noqa: E402
 from galoisgpu_amd.batch import DeviceArray, sssp_graphs, sssp_multi_into  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
@@ -239,7 +241,29 @@ def c5(repeats: int) -> dict:
     return out
 
 
-CONFIGS = {"C1": c1, "C2": c2, "C3": c3, "C4B": c4b, "C5": c5}
+def cc_graphs(repeats: int) -> dict:
+    """Connected components (apps/cc) on the RMAT-22 and road-like inputs of the
+    SSSP configurations; the CPU side is the oracle's union-find, one core."""
+    out = {"config": "CC"}
+    for name, g in (("rmat22", CSRGraph.rmat(22, 16, 0.57, 0.19, 0.19, 1, 255, seed=1)),
+                    ("road2048", CSRGraph.roadlike(2048, 2048, 0.10, 0.001, 1, 10000, seed=1))):
+        t = time.perf_counter()
+        ref, k = oracle.cc(g.row_start, g.edge_dst)
+        t_cpu = time.perf_counter() - t
+        dev = g.copy_to_gpu()
+        lab = DeviceArray(g.nnodes * 4)
+        st = cc_into(dev, lab.ptr)
+        assert np.array_equal(lab.to_numpy(np.int32, g.nnodes), ref), f"CC {name} differs"
+        ms = sorted(cc_into(dev, lab.ptr)["ms_kernel"] for _ in range(repeats))[repeats // 2]
+        out[name] = {"V": g.nnodes, "E": g.nedges, "components": k, "passes": st["iterations"], "ms": round(ms, 4),
+                     "gedges_per_s": round(g.nedges / ms / 1e6, 3),
+                     "cpu": {"ms": round(t_cpu * 1e3, 2), "cores": 1, "kind": "port (oracle/oracle.c union-find)"}}
+        lab.close()
+        dev.close()
+    return out
+
+
+CONFIGS = {"C1": c1, "C2": c2, "C3": c3, "C4B": c4b, "C5": c5, "CC": cc_graphs}
 
 
 def main() -> int:

@@ -174,3 +174,70 @@ def test_pins_catch_planted_bugs(small_graphs, bug):
         assert failures, f"planted bug {bug} was not caught by any pin"
     finally:
         oracle.plant_bug(0)
+
+
+# ---- connected components (apps/cc) ------------------------------------------
+
+def min_id_labels(n, row, col):
+    """Weak components from scipy (an independent library), relabelled by the
+    smallest vertex id of each component."""
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import connected_components
+    m = csr_matrix((np.ones(len(col)), np.asarray(col, np.int64), np.asarray(row, np.int64)), shape=(n, n))
+    k, lab = connected_components(m, directed=True, connection="weak")
+    first = np.full(k, n, dtype=np.int64)
+    np.minimum.at(first, lab, np.arange(n))
+    return first[lab].astype(np.int32), k
+
+
+def random_csr(n, m, seed):
+    rng = np.random.default_rng(seed)
+    src = rng.integers(0, n, m)
+    dst = rng.integers(0, n, m)
+    order = np.argsort(src, kind="stable")
+    row = np.zeros(n + 1, dtype=np.uint32)
+    row[1:] = np.cumsum(np.bincount(src, minlength=n))
+    return row, dst[order].astype(np.uint32)
+
+
+def test_cc_matches_scipy(small_graphs):
+    for name, g in small_graphs.items():
+        lab, k = oracle.cc(g["row"], g["col"])
+        ref, rk = min_id_labels(len(g["row"]) - 1, g["row"], g["col"])
+        assert np.array_equal(lab, ref) and k == rk, name
+    for seed, (n, m) in enumerate([(50, 30), (300, 200), (2000, 1500), (5000, 9000)]):
+        row, col = random_csr(n, m, seed)
+        lab, k = oracle.cc(row, col)
+        ref, rk = min_id_labels(n, row, col)
+        assert np.array_equal(lab, ref) and k == rk, (n, m)
+
+
+def test_cc_closed_forms():
+    row, col, _ = grid_csr(9, 13)
+    lab, k = oracle.cc(row, col)
+    assert k == 1 and not lab.any()
+    # edges only from larger to smaller ids still join everything (directions ignored)
+    n = 40
+    col = np.arange(0, n - 1, dtype=np.uint32)  # vertex i+1 -> i
+    row = np.concatenate([[0, 0], np.arange(1, n, dtype=np.uint32)]).astype(np.uint32)
+    lab, k = oracle.cc(row, col)
+    assert k == 1 and not lab.any()
+    # isolated vertices and two paths: labels are the first vertex of each part
+    row = np.array([0, 1, 2, 2, 2, 3, 4, 4], np.uint32)  # 0->1->2, 3 alone, 4->5->6
+    col = np.array([1, 2, 5, 6], np.uint32)
+    lab, k = oracle.cc(row, col)
+    assert k == 3 and lab.tolist() == [0, 0, 0, 3, 4, 4, 4]
+
+
+def test_cc_pins_catch_planted_bug():
+    oracle.plant_bug(4)
+    try:
+        caught = False
+        for seed, (n, m) in enumerate([(300, 200), (2000, 1500)]):
+            row, col = random_csr(n, m, seed)
+            lab, k = oracle.cc(row, col)
+            ref, rk = min_id_labels(n, row, col)
+            caught = caught or not np.array_equal(lab, ref)
+        assert caught, "skipping the last edge of each row was not caught"
+    finally:
+        oracle.plant_bug(0)
