@@ -163,6 +163,9 @@ SIGNATURES = {
     "lsg_cc": (c_int, [POINTER(LsgGraph), c_void_p, POINTER(c_uint32), POINTER(LsgStats)]),
     "lsg_cc_gg_main": (c_int, [POINTER(LsgGraph), POINTER(LsgGraph), POINTER(LsgStats)]),
     "lsg_output_cc": (c_int, [POINTER(LsgGraph), c_char_p]),
+    "lsg_pagerank": (c_int, [POINTER(LsgGraph), POINTER(LsgGraph), c_float, c_float, c_int32, c_void_p,
+                             POINTER(LsgStats)]),
+    "lsg_output_pr": (c_int, [c_void_p, c_uint32, c_char_p, c_int]),
     "lsg_bfs_do": (c_int, [POINTER(LsgGraph), POINTER(LsgGraph), c_uint32, c_int, c_void_p, POINTER(LsgStats)]),
     "lsg_output": (c_int, [POINTER(LsgGraph), c_char_p]),
     "lsg_output_dist": (c_int, [c_void_p, c_uint32, c_char_p]),

@@ -110,6 +110,18 @@ def cc(graph: CSRGraph, device: int = 0) -> Tuple[np.ndarray, dict]:
     return _run(graph, Options(), device, run)
 
 
+PR_ALPHA, PR_EPSILON = 0.85, 1e-6  # apps/pr/kernel.cu:19-20
+
+
+def pagerank_into(dev: DeviceGraph, rev: DeviceGraph, rank_ptr: int, alpha: float = PR_ALPHA,
+                  epsilon: float = PR_EPSILON, max_iterations: int = 2 ** 31 - 1) -> dict:
+    """PageRank (lsg_pagerank) into a device float array; `rev` is dev.transpose()."""
+    st = LsgStats()
+    check(lib().lsg_pagerank(ctypes.byref(dev._g), ctypes.byref(rev._g), alpha, epsilon, max_iterations,
+                             ctypes.c_void_p(rank_ptr), ctypes.byref(st)), "pagerank")
+    return st.as_dict()
+
+
 BFS_AUTO, BFS_TOP_DOWN, BFS_BOTTOM_UP = 0, 1, 2
 
 

@@ -47,9 +47,6 @@ namespace lsg {
 #define LSG_STEAL 1
 #endif
 // Waves per SIMD the register allocation must allow (0: compiler's choice).
-#ifndef LSG_SPARSE_DEFER
-#define LSG_SPARSE_DEFER 0
-#endif
 #ifndef LSG_MULTI_WAVES
 #define LSG_MULTI_WAVES 4
 #endif
@@ -370,17 +367,6 @@ __device__ __forceinline__ void process_items(const MArgs &a, const Sched &s, La
     rs = a.row_start[u];
     re = a.row_start[u + 1];
   }
-#if LSG_SPARSE_DEFER
-  // Sparse items wait a round to gather more sources: in odd rounds of a
-  // large frontier an item with few active sources puts its bits back for the
-  // next round (where it is expanded whatever its count).
-  if ((s.round & 1) && s.in_count >= a.nnodes / 16) {
-    const bool sparse = m != 0 && __builtin_popcountll(m) < LSG_SPARSE_DEFER;
-    const bool push = sparse && atomic_or64(a.nmask[s.round & 1] + u, m) == 0 && near_admit(a, s, u);
-    wave_push(push, u, a.q[s.near_q], &a.ctrl->qcount[s.near_q], a.qcap, &a.ctrl->abort);
-    if (sparse) m = 0;
-  }
-#endif
   // The row loads depend on the exchanged masks, so each reads a distance at
   // least as new as the improvements that set the consumed bits.
   uint32_t du[kItems];

@@ -380,6 +380,24 @@ int lsg_output_cc(const lsg_graph *g, const char *path) {
   return LSG_OK;
 }
 
+int lsg_output_pr(const float *rank, uint32_t n, const char *path, int ids_only) {
+  if (!rank) return fail(LSG_ERR_ARG, "output: NULL ranks");
+  float sum = 0;  // apps/pr/support.cu:57-62: a float running sum in node order
+  for (uint32_t i = 0; i < n; ++i) sum += rank[i];
+  fprintf(stdout, "sum: %f (%d)\n", sum, (int)n);
+  if (!path) return LSG_OK;
+  bool own = strcmp(path, "-") != 0;
+  FILE *f = own ? fopen(path, "w") : stdout;
+  if (!f) return fail(LSG_ERR_IO, "unable to open %s for writing: %s", path, strerror(errno));
+  bool ok = true;
+  for (uint32_t i = 0; i < n && ok; ++i)  // support.cu:91-96
+    ok = (ids_only ? fprintf(f, "%d\n", (int)i) : fprintf(f, "%d %f\n", (int)i, (double)rank[i])) > 0;
+  if (own) ok = (fclose(f) == 0) && ok;
+  else fflush(f);
+  if (!ok) return fail(LSG_ERR_IO, "write to %s failed", path);
+  return LSG_OK;
+}
+
 uint32_t lsg_max_degree_vertex(const lsg_graph *g) {
   if (!g || !g->row_start || g->device_graph) return 0;
   uint32_t best = 0, bd = 0;

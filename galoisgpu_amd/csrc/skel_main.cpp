@@ -1,4 +1,5 @@
-// The sssp / bfs command line, built twice (-DLSG_APP_SSSP / -DLSG_APP_BFS).
+// The sssp / bfs / cc / pr command line, one binary per app (-DLSG_APP_SSSP,
+// -DLSG_APP_BFS, -DLSG_APP_CC, -DLSG_APP_PR).
 //
 // Mirrors the reference's skeleton driver and app options:
 //   skelapp/skel.cu:80-151   usage, getopt "g:qo:" + app options, main
@@ -20,6 +21,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <climits>
+#include <vector>
 
 #include "lsg.h"
 
@@ -35,10 +38,14 @@ static const char *GGC_OPTIONS = "lsg-mi355x bfs: persistent level-synchronous, 
 static const char *prog_opts = "";
 static const char *prog_usage = "";
 static const char *GGC_OPTIONS = "lsg-mi355x cc: hooking + pointer jumping, wave64, gfx950";
+#elif defined(LSG_APP_PR)
+static const char *prog_opts = "nt:x:";  // apps/pr/support.cu:15-17
+static const char *prog_usage = "[-n] [-t top_ranks] [-x max_iterations]";
+static const char *GGC_OPTIONS = "lsg-mi355x pr: pull Jacobi over the reverse CSR, wave64, gfx950";
 #else
-#error "define LSG_APP_SSSP, LSG_APP_BFS or LSG_APP_CC"
+#error "define LSG_APP_SSSP, LSG_APP_BFS, LSG_APP_CC or LSG_APP_PR"
 #endif
-#if defined(LSG_APP_CC)
+#if defined(LSG_APP_CC) || defined(LSG_APP_PR)
 static const char *prog_args_usage = "";
 #elif defined(LSG_APP_BFS)
 static const char *prog_args_usage =
@@ -48,6 +55,10 @@ static const char *prog_args_usage =
 static const char *prog_args_usage = "-l: enable thread block load balancer (by default false)";
 #endif
 static int DIRECTION_OPT = 0;  // bfs -D
+#if defined(LSG_APP_PR)
+static int NO_PRINT_PAGERANK = 0;     // pr -n (apps/pr/support.cu:23-24)
+static int MAX_ITERATIONS = INT_MAX;  // pr -x (apps/pr/support.cu:25)
+#endif
 
 static int QUIET = 0;
 static char *INPUT = nullptr, *OUTPUT = nullptr;
@@ -74,6 +85,11 @@ static void process_prog_opt(char c, char *optarg) {
   }
   if (c == 'l') OPTS.enable_lb = 1;
   if (c == 'D') DIRECTION_OPT = 1;
+#if defined(LSG_APP_PR)
+  if (c == 'n') NO_PRINT_PAGERANK = 1;
+  if (c == 'x') MAX_ITERATIONS = atoi(optarg);
+  if (c == 't') (void)atoi(optarg);  // accepted; the reference's top-ranks listing is disabled (support.cu:83-89)
+#endif
   if (c == 's') {
     OPTS.start_node = atoi(optarg);
     if (OPTS.start_node < 0) {
@@ -140,10 +156,22 @@ static int load_graph_and_run_kernel(char *graph_file) {
   }
 #endif
   uint64_t t0 = now_ns();
+#if defined(LSG_APP_PR)
+  // gg_main of apps/pr: ranks in their own float array (P_CURR), the reverse
+  // CSR built inside the timed region
+  float *rank = nullptr;
+  lsg_graph gr;
+  memset(&gr, 0, sizeof gr);
+  rc = lsg_dev_malloc(CUDA_DEVICE, (size_t)gg.nnodes * sizeof(float), (void **)&rank);
+  if (rc) die("allocating ranks", rc);
+#endif
 #if defined(LSG_APP_SSSP)
   rc = lsg_sssp_gg_main(&g, &gg, &OPTS, &st);
 #elif defined(LSG_APP_CC)
   rc = lsg_cc_gg_main(&g, &gg, &st);
+#elif defined(LSG_APP_PR)
+  rc = lsg_graph_transpose(&gg, &gr);
+  if (rc == LSG_OK) rc = lsg_pagerank(&gg, &gr, 0.85f, 0.000001f, MAX_ITERATIONS, rank, &st);  // kernel.cu:19-20
 #else
   if (DIRECTION_OPT)  // lsg_bfs_do rejects a source out of range
     rc = lsg_bfs_do(&gg, &gt, (uint32_t)OPTS.start_node, 0, gg.node_data, &st);
@@ -157,16 +185,28 @@ static int load_graph_and_run_kernel(char *graph_file) {
   printf("iterations: %d\n", (int)st.iterations);  // apps/sssp/kernel.cu:479
 #elif defined(LSG_APP_CC)
   printf("components: %d\n", (int)st.items);  // apps/cc/kernel.cu:719
+#elif defined(LSG_APP_PR)
+  printf("PR took %d iterations\n", (int)st.iterations);  // apps/pr/kernel.cu:387
 #endif
   printf("gg_main %llu %llu\n", (unsigned long long)t0, (unsigned long long)t1);  // Timer.h:77-79
   fprintf(stderr, "Total time: %llu ms\n", (unsigned long long)((t1 - t0) / 1000000ull));
   fprintf(stderr, "Total time: %llu ns\n", (unsigned long long)(t1 - t0));
+#if defined(LSG_APP_PR)
+  std::vector<float> hrank(gg.nnodes);
+  rc = lsg_dev_copy(CUDA_DEVICE, hrank.data(), rank, hrank.size() * sizeof(float), 1);
+  if (rc) die("copying ranks", rc);
+  lsg_dev_free(CUDA_DEVICE, rank);
+  lsg_graph_dealloc(&gr);
+#else
   rc = lsg_graph_copy_to_cpu(&gg, &g);
   if (rc) die("copy_to_cpu", rc);
+#endif
   if (!QUIET) {
     fflush(stdout);
 #if defined(LSG_APP_CC)
     rc = lsg_output_cc(&g, OUTPUT);
+#elif defined(LSG_APP_PR)
+    rc = lsg_output_pr(hrank.data(), gg.nnodes, OUTPUT, NO_PRINT_PAGERANK);
 #else
     rc = lsg_output(&g, OUTPUT);
 #endif

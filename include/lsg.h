@@ -206,6 +206,20 @@ int lsg_cc_gg_main(lsg_graph *hg, lsg_graph *gg, lsg_stats *stats);
  * edge "\te %d: %d %d %d" (edge id, destination, weight, destination's label);
  * file "-" is stdout. */
 int lsg_output_cc(const lsg_graph *host_g, const char *file);
+/* PageRank (apps/pr/kernel.cu): the fixpoint of
+ *   p[v] = (1 - alpha) + alpha * sum over in-edges (u, v) of p[u] / outdeg(u)
+ * by pull-based Jacobi iteration over `gt`, the reverse of `g`
+ * (lsg_graph_transpose), until no rank changes by more than epsilon times
+ * itself or max_iterations pass.  The reference uses alpha 0.85, epsilon 1e-6
+ * (apps/pr/kernel.cu:19-20) and pushes residuals instead; both reach the same
+ * fixpoint, within float rounding.  rank_dev is a device array of nnodes
+ * floats.  stats: iterations. */
+int lsg_pagerank(const lsg_graph *g, const lsg_graph *gt, float alpha, float epsilon, int32_t max_iterations,
+                 float *rank_dev, lsg_stats *stats);
+/* apps/pr/support.cu:45-97 output: prints "sum: %f (%d)" (float sum of the
+ * ranks, node count) on stdout, then, if file is not NULL, one line per node:
+ * "%d %f" (id, rank), or "%d" with ids_only (-n); file "-" is stdout. */
+int lsg_output_pr(const float *rank, uint32_t n, const char *file, int ids_only);
 /* Direction-optimising BFS from `src` (no reference counterpart as an
  * algorithm; the levels are apps/bfs's, as lsg_bfs_dist writes them).  `gt`
  * is the reverse of `g` (lsg_graph_transpose).  direction: 0 = per level by

@@ -1,4 +1,4 @@
-"""CPU oracle for the SSSP/BFS/CC paths -- TEST INFRASTRUCTURE ONLY.
+"""CPU oracle for the SSSP/BFS/CC/PR paths -- TEST INFRASTRUCTURE ONLY.
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
 import this package, and only as the checker.  It wraps oracle/liboracle.so
@@ -47,6 +47,8 @@ def lib() -> ctypes.CDLL:
     l.oracle_sssp_certificate.restype = i64
     l.oracle_reach.argtypes = [u32, P, P, P, P]
     l.oracle_reach.restype = None
+    l.oracle_pagerank.argtypes = [u32, P, P, ctypes.c_double, ctypes.c_double, ctypes.c_int, P]
+    l.oracle_pagerank.restype = ctypes.c_int
     l.oracle_cc.argtypes = [u32, P, P, P]
     l.oracle_cc.restype = i64
     l.oracle_plant_bug.argtypes = [ctypes.c_int]
@@ -128,6 +130,17 @@ def cc(row_start, edge_dst) -> Tuple[np.ndarray, int]:
     if k < 0:
         raise ValueError("oracle_cc failed")
     return lab, int(k)
+
+
+def pagerank(row_start, edge_dst, alpha: float = 0.85, tol: float = 1e-13, max_iter: int = 100000) -> Tuple[np.ndarray, int]:
+    """PageRank fixpoint of apps/pr in double: (ranks, Jacobi iterations)."""
+    r, c, _ = _arrs(row_start, edge_dst)
+    n = len(r) - 1
+    p = np.empty(n, dtype=np.float64)
+    it = lib().oracle_pagerank(n, _p(r), _p(c), alpha, tol, max_iter, _p(p))
+    if it < 0:
+        raise ValueError("oracle_pagerank failed")
+    return p, int(it)
 
 
 def plant_bug(which: int) -> None:

@@ -19,6 +19,8 @@
  *   oracle_cc          weakly connected components, labelled by the smallest
  *                      id of each component (apps/cc/kernel.cu:630-723's
  *                      partition; union-find)
+ *   oracle_pagerank    the fixpoint apps/pr/kernel.cu converges to, by Jacobi
+ *                      iteration in double
  *   oracle_sssp_certificate  O(E) shortest-path certificate: d[src] = 0,
  *                      every edge out of a reached vertex is relaxed, every
  *                      reached vertex other than the source has a tight
@@ -318,4 +320,44 @@ int64_t oracle_cc(uint32_t n, const uint32_t *row, const uint32_t *col, int32_t 
   }
   free(p);
   return comps;
+}
+
+/* PageRank fixpoint (apps/pr/kernel.cu:19-20,38,110-236 reach it by pushing
+ * residuals):  p[v] = (1 - alpha) + alpha * sum over edges (u, v) of
+ * p[u] / outdeg(u), by Jacobi iteration in double until the largest change
+ * falls below tol or max_iter passes.  Returns the iterations taken.
+ * Planted mistake 5 (tests only): the out-degree of every vertex is taken
+ * one too large. */
+int oracle_pagerank(uint32_t n, const uint32_t *row, const uint32_t *col, double alpha, double tol, int max_iter,
+                    double *p) {
+  double *q = (double *)malloc((size_t)n * sizeof(double));
+  double *c = (double *)malloc((size_t)n * sizeof(double));
+  if (!q || !c) {
+    free(q);
+    free(c);
+    return -1;
+  }
+  for (uint32_t v = 0; v < n; ++v) p[v] = 1.0 - alpha;
+  int it = 0;
+  while (it < max_iter) {
+    for (uint32_t u = 0; u < n; ++u) {
+      const uint32_t od = row[u + 1] - row[u] + (g_plant == 5);
+      c[u] = od ? p[u] / (double)od : 0.0;
+    }
+    for (uint32_t v = 0; v < n; ++v) q[v] = 0.0;
+    for (uint32_t u = 0; u < n; ++u)
+      for (uint32_t e = row[u]; e < row[u + 1]; ++e) q[col[e]] += c[u];
+    double change = 0.0;
+    for (uint32_t v = 0; v < n; ++v) {
+      const double pn = (1.0 - alpha) + alpha * q[v];
+      const double d = pn > p[v] ? pn - p[v] : p[v] - pn;
+      if (d > change) change = d;
+      p[v] = pn;
+    }
+    ++it;
+    if (change < tol) break;
+  }
+  free(q);
+  free(c);
+  return it;
 }

@@ -16,6 +16,7 @@ oracle's serial near/far restatement timed beside it on the host.
   C5  RMAT-24 ef16, 32 sources: batched (+ learned) vs plain per source
   CC  connected components (apps/cc) of the RMAT-22 and 2048x2048 road-like
       graphs against the oracle's union-find
+  PR  PageRank (apps/pr) of the same graphs against the oracle's double fixpoint
 
 The headline RMAT-22 x 64-source configuration is bench.py.  This file uses
 the oracle (test infrastructure) as the checker and CPU baseline, which is why
@@ -41,7 +42,7 @@ sys.path.insert(0, ROOT)
 import oracle  # noqa: E402
 from galoisgpu_amd import CSRGraph, Options  # noqa: E402
 from galoisgpu_amd import sched as lsched  # noqa: E402
-from galoisgpu_amd.apps import bfs_do_into, bfs_into, cc_into, sssp_into  # noqa: E402
+from galoisgpu_amd.apps import bfs_do_into, bfs_into, cc_into, pagerank_into, sssp_into  # noqa: E402
 from galoisgpu_amd.batch import DeviceArray, sssp_graphs, sssp_multi_into  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
@@ -263,7 +264,37 @@ def cc_graphs(repeats: int) -> dict:
     return out
 
 
-CONFIGS = {"C1": c1, "C2": c2, "C3": c3, "C4B": c4b, "C5": c5, "CC": cc_graphs}
+def pr_graphs(repeats: int) -> dict:
+    """PageRank (apps/pr, alpha 0.85, epsilon 1e-6) on the RMAT-22 and road-like
+    inputs: GPU float32 against the oracle's double fixpoint (largest relative
+    error reported); the CPU side is the oracle's Jacobi iteration timed per
+    iteration over 3 iterations, one core."""
+    out = {"config": "PR"}
+    for name, g in (("rmat22", CSRGraph.rmat(22, 16, 0.57, 0.19, 0.19, 1, 255, seed=1)),
+                    ("road2048", CSRGraph.roadlike(2048, 2048, 0.10, 0.001, 1, 10000, seed=1))):
+        dev = g.copy_to_gpu()
+        rev = dev.transpose()
+        rank = DeviceArray(g.nnodes * 4)
+        st = pagerank_into(dev, rev, rank.ptr)
+        got = rank.to_numpy(np.float32, g.nnodes).astype(np.float64)
+        ms = sorted(pagerank_into(dev, rev, rank.ptr)["ms_kernel"] for _ in range(repeats))[repeats // 2]
+        ref, _ = oracle.pagerank(g.row_start, g.edge_dst, tol=1e-9, max_iter=400)
+        err = float((np.abs(got - ref) / ref).max())
+        assert err <= 1e-4, f"PR {name} relative error {err}"
+        t = time.perf_counter()
+        _, it = oracle.pagerank(g.row_start, g.edge_dst, max_iter=3)
+        cpu_iter = (time.perf_counter() - t) / it
+        out[name] = {"V": g.nnodes, "E": g.nedges, "iterations": st["iterations"], "ms": round(ms, 3),
+                     "ms_per_iteration": round(ms / st["iterations"], 4), "max_rel_error": err,
+                     "cpu": {"ms_per_iteration": round(cpu_iter * 1e3, 2), "cores": 1,
+                             "kind": "port (oracle/oracle.c Jacobi, double)"}}
+        rank.close()
+        rev.close()
+        dev.close()
+    return out
+
+
+CONFIGS = {"C1": c1, "C2": c2, "C3": c3, "C4B": c4b, "C5": c5, "CC": cc_graphs, "PR": pr_graphs}
 
 
 def main() -> int:

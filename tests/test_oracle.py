@@ -241,3 +241,48 @@ def test_cc_pins_catch_planted_bug():
         assert caught, "skipping the last edge of each row was not caught"
     finally:
         oracle.plant_bug(0)
+
+
+# ---- PageRank (apps/pr) --------------------------------------------------------
+
+def pagerank_solve(n, row, col, alpha=0.85):
+    """The same fixpoint by a dense linear solve (numpy), an independent method:
+    (I - alpha M^T) p = (1 - alpha) 1 with M[u, v] = multiplicity(u -> v) / outdeg(u)."""
+    row = np.asarray(row, np.int64)
+    M = np.zeros((n, n))
+    for u in range(n):
+        od = row[u + 1] - row[u]
+        for e in range(row[u], row[u + 1]):
+            M[u, col[e]] += 1.0 / od
+    return np.linalg.solve(np.eye(n) - alpha * M.T, np.full(n, 1.0 - alpha))
+
+
+def test_pagerank_closed_forms():
+    n = 17
+    cyc_row = np.arange(n + 1, dtype=np.uint32)
+    cyc_col = ((np.arange(n) + 1) % n).astype(np.uint32)
+    p, _ = oracle.pagerank(cyc_row, cyc_col)
+    assert np.allclose(p, 1.0, rtol=0, atol=1e-12)  # p = 0.15 + 0.85 p
+    p, _ = oracle.pagerank(np.zeros(6, np.uint32), np.zeros(0, np.uint32))
+    assert np.allclose(p, 0.15, rtol=0, atol=1e-15)
+    star_row = np.array([0, n - 1] + [n - 1] * (n - 1), np.uint32)  # 0 -> 1..n-1
+    star_col = np.arange(1, n, dtype=np.uint32)
+    p, _ = oracle.pagerank(star_row, star_col)
+    assert abs(p[0] - 0.15) < 1e-15 and np.allclose(p[1:], 0.15 + 0.85 * 0.15 / (n - 1), rtol=0, atol=1e-14)
+
+
+def test_pagerank_matches_linear_solve():
+    for seed, (n, m) in enumerate([(30, 60), (200, 700), (400, 300)]):  # duplicates, dangling vertices
+        row, col = random_csr(n, m, seed)
+        p, _ = oracle.pagerank(row, col)
+        assert np.allclose(p, pagerank_solve(n, row, col), rtol=1e-10, atol=0), (n, m)
+
+
+def test_pagerank_pins_catch_planted_bug():
+    oracle.plant_bug(5)
+    try:
+        row, col = random_csr(200, 700, 1)
+        p, _ = oracle.pagerank(row, col)
+        assert not np.allclose(p, pagerank_solve(200, row, col), rtol=1e-6, atol=0)
+    finally:
+        oracle.plant_bug(0)
