@@ -13,8 +13,10 @@
 // The host picks the direction per level from the frontier's size and edge
 // count (Beamer, Asanovic and Patterson's rule: bottom-up once the frontier's
 // edges exceed 1/alpha of the unexplored edges, top-down again once the
-// frontier holds fewer than 1/beta of the vertices).  Both steps push the
-// vertices they discover, so the next level always has its queue.
+// frontier holds fewer than 1/beta of the vertices).  Every level marks the
+// vertices it discovers in a bitmap; a top-down level also queues them, and
+// a top-down level after a bottom-up one compacts that level's bitmap into
+// its queue first.
 
 #include <hip/hip_runtime.h>
 
@@ -236,12 +238,28 @@ __global__ __launch_bounds__(kDoBlock) void top_down_kernel(DoArgs a, const uint
 // are L, never L-1, so racing reads cannot mislead the search.  (Measured on
 // RMAT-20: 8 lanes per vertex with coalesced in-edge reads was 4x slower --
 // each wavefront then searches 8 vertices instead of 64.)
+//
+// A wavefront owns 64 consecutive vertices (64-aligned), so it writes their
+// two words of this level's bitmap with plain stores -- every word of the
+// bitmap is written, and no atomics are needed.  The level's vertex count and
+// out-edge count are summed per block into partial[2 b], partial[2 b + 1]
+// (bu_reduce_kernel adds them up): one counter that every wavefront with a
+// discovery bumps serialises the whole level (RMAT-20: 2 x 16K same-address
+// atomics took ~240 us per level).  Nothing is queued: a top-down level that
+// follows compacts the bitmap into its queue (bitmap_queue_kernel).
 constexpr uint32_t kBuRun = 4;
-__global__ __launch_bounds__(kDoBlock) void bottom_up_kernel(DoArgs a) {
+__global__ __launch_bounds__(kDoBlock) void bottom_up_kernel(DoArgs a, uint32_t *partial) {
+  __shared__ uint32_t s_cnt, s_deg;
+  if (threadIdx.x == 0) s_cnt = s_deg = 0;
   clear_next(a);
+  __syncthreads();
   const uint32_t *prev = a.fb[(a.L - 1) % 3];
+  uint32_t *cur = a.fb[a.L % 3];
+  const uint32_t words = (a.nnodes + 31) / 32;
+  const uint32_t lane = lane_id();
   const size_t nt = (size_t)gridDim.x * blockDim.x;
-  // whole wavefronts iterate together so that the push's ballot sees them all
+  uint32_t cnt = 0, deg = 0;
+  // whole wavefronts iterate together: lane 0's vertex is the first of the 64
   const size_t vend = (a.nnodes + (size_t)kWave - 1) / kWave * kWave;
   for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < vend; v += nt) {
     bool found = false;
@@ -255,10 +273,82 @@ __global__ __launch_bounds__(kDoBlock) void bottom_up_kernel(DoArgs a) {
         for (uint32_t k = 0; k < kBuRun; ++k)
           found = found || (u[k] < a.nnodes && ((prev[u[k] >> 5] >> (u[k] & 31)) & 1u));
       }
-      if (found) a.level[v] = a.L;
+      if (found) {
+        a.level[v] = a.L;
+        cnt += 1;
+        deg += a.row_start[v + 1] - a.row_start[v];
+      }
     }
-    do_push(a, found, (uint32_t)v);
+    const uint64_t m = __ballot(found);
+    if (lane == 0) {
+      const uint32_t w0 = (uint32_t)(v / 32);  // v = the wave's first vertex, < nnodes
+      cur[w0] = (uint32_t)m;
+      if (w0 + 1 < words) cur[w0 + 1] = (uint32_t)(m >> 32);
+    }
   }
+#pragma unroll
+  for (int d = kWave / 2; d >= 1; d >>= 1) {
+    cnt += __shfl_xor(cnt, d);
+    deg += __shfl_xor(deg, d);
+  }
+  if (lane == 0 && cnt) {
+    atomicAdd(&s_cnt, cnt);
+    atomicAdd(&s_deg, deg);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = s_cnt;
+    partial[2 * blockIdx.x + 1] = s_deg;
+  }
+}
+
+// One block: out[0] = sum of partial[2 b], out[1] = sum of partial[2 b + 1]
+// over n blocks.
+__global__ __launch_bounds__(kDoBlock) void bu_reduce_kernel(const uint32_t *partial, uint32_t n, uint32_t *out) {
+  __shared__ uint32_t s[2];
+  if (threadIdx.x < 2) s[threadIdx.x] = 0;
+  __syncthreads();
+  uint32_t c = 0, d = 0;
+  for (uint32_t b = threadIdx.x; b < n; b += blockDim.x) {
+    c += partial[2 * b];
+    d += partial[2 * b + 1];
+  }
+#pragma unroll
+  for (int k = kWave / 2; k >= 1; k >>= 1) {
+    c += __shfl_xor(c, k);
+    d += __shfl_xor(d, k);
+  }
+  if (lane_id() == 0) {
+    atomicAdd(&s[0], c);
+    atomicAdd(&s[1], d);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    out[0] = s[0];
+    out[1] = s[1];
+  }
+}
+
+// Queue of the vertices whose bit is set in bitmap fb (words 0 .. words-1):
+// lane i takes word i of a 64-word run, and one atomic per wavefront reserves
+// the run's places in q.
+__global__ void bitmap_queue_kernel(const uint32_t *fb, uint32_t words, uint32_t *q, uint32_t *tail) {
+  const size_t nt = (size_t)gridDim.x * blockDim.x;
+  const size_t iend = ((size_t)words + kWave - 1) / kWave * kWave;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < iend; i += nt) {
+    uint32_t bits = i < words ? fb[i] : 0u;
+    const uint32_t n = (uint32_t)__builtin_popcount(bits);
+    uint32_t pos = wave_reserve(n, tail);
+    while (bits) {
+      const uint32_t b = (uint32_t)__builtin_ctz(bits);
+      q[pos++] = (uint32_t)i * 32 + b;
+      bits &= bits - 1;
+    }
+  }
+}
+
+__global__ void zero_words_kernel(uint32_t *p, uint32_t n) {
+  if (threadIdx.x < n) p[threadIdx.x] = 0;
 }
 
 // level = INF except src = 0; bitmaps 0 and 1 cleared (2 is cleared by
@@ -354,17 +444,18 @@ int lsg_bfs_do(const lsg_graph *g, const lsg_graph *gt, uint32_t src, int direct
   void *qbuf, *cbuf, *pbuf;
   const uint32_t ptiles = (V + kScanTile - 1) / kScanTile;
   const size_t words = ((size_t)V + 31) / 32;
+  const uint32_t vgrid = (uint32_t)std::min<uint64_t>(((uint64_t)V + 255) / 256, 16ull * w->cus);
   if ((rc = ws_buffer(w, kBufQueues, 2ull * V * 4 + 3 * words * 4, &qbuf)) || (rc = ws_buffer(w, kBufSeeds, 2048, &cbuf)) ||
-      (rc = ws_buffer(w, kBufSeedList, ((size_t)V + ptiles + 64) * 4, &pbuf)))
+      (rc = ws_buffer(w, kBufSeedList, ((size_t)V + ptiles + 64 + 2ull * vgrid) * 4, &pbuf)))
     return rc;
   uint32_t *q[2] = {(uint32_t *)qbuf, (uint32_t *)qbuf + V};
   uint32_t *fbits = (uint32_t *)qbuf + 2ull * V;
-  uint32_t *ctr = (uint32_t *)cbuf;
+  uint32_t *ctr = (uint32_t *)cbuf;  // [0..3] level counts by parity, [4] queue tail of a compaction
   uint32_t *pre = (uint32_t *)pbuf, *ptile = pre + V + 1, *ptotal = ptile + ptiles;
+  uint32_t *partial = pre + V + ptiles + 64;  // per-block counts of a bottom-up level
   uint32_t rs2[2];
   HIPCHK(hipMemcpy(rs2, g->row_start + src, 8, hipMemcpyDeviceToHost));
   HIPCHK(hipEventRecord(w->ev0, w->stream));
-  const uint32_t vgrid = (uint32_t)std::min<uint64_t>(((uint64_t)V + 255) / 256, 16ull * w->cus);
   hipLaunchKernelGGL(do_init_kernel, dim3(vgrid), dim3(256), 0, w->stream, (uint32_t *)level_dev, V, src, q[0], ctr,
                      fbits, fbits + words);
   DoArgs a;
@@ -378,7 +469,7 @@ int lsg_bfs_do(const lsg_graph *g, const lsg_graph *gt, uint32_t src, int direct
   a.nnodes = V;
   uint64_t m_u = g->nedges, m_f = rs2[1] - rs2[0];
   uint32_t n_in = 1, L = 1, bu_levels = 0;
-  bool bottom_up = direction == 2;
+  bool bottom_up = direction == 2, prev_bottom_up = false;
   while (n_in > 0) {
     if (direction == 0) {
       // Beamer's rule: go bottom-up when the frontier's edges exceed the
@@ -391,9 +482,18 @@ int lsg_bfs_do(const lsg_graph *g, const lsg_graph *gt, uint32_t src, int direct
     a.n_in = n_in;
     a.L = L;
     if (bottom_up) {
-      hipLaunchKernelGGL(bottom_up_kernel, dim3(vgrid), dim3(kDoBlock), 0, w->stream, a);
+      hipLaunchKernelGGL(bottom_up_kernel, dim3(vgrid), dim3(kDoBlock), 0, w->stream, a, partial);
+      hipLaunchKernelGGL(bu_reduce_kernel, dim3(1), dim3(kDoBlock), 0, w->stream, (const uint32_t *)partial, vgrid,
+                         ctr + 2 * (L & 1));
       ++bu_levels;
     } else {
+      if (prev_bottom_up) {
+        // a bottom-up level leaves its vertices in its bitmap only
+        const uint32_t wgrid = (uint32_t)std::min<uint64_t>((words + 255) / 256, 16ull * w->cus);
+        hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, w->stream, ctr + 4, 1u);
+        hipLaunchKernelGGL(bitmap_queue_kernel, dim3(wgrid), dim3(256), 0, w->stream,
+                           (const uint32_t *)a.fb[(L - 1) % 3], (uint32_t)words, q[(L - 1) & 1], ctr + 4);
+      }
       const uint32_t nt = (n_in + kScanTile - 1) / kScanTile;
       const uint32_t g1 = (uint32_t)std::min<uint64_t>(((uint64_t)n_in + 255) / 256, 16ull * w->cus);
       hipLaunchKernelGGL(frontier_deg_kernel, dim3(g1), dim3(256), 0, w->stream, a.q_in, n_in, a.row_start, pre);
@@ -408,6 +508,7 @@ int lsg_bfs_do(const lsg_graph *g, const lsg_graph *gt, uint32_t src, int direct
     uint32_t *c = w->pinned;  // pinned: the per-level readback is one short DMA
     HIPCHK(hipMemcpyAsync(c, ctr + 2 * (L & 1), 8, hipMemcpyDeviceToHost, w->stream));
     HIPCHK(hipStreamSynchronize(w->stream));
+    prev_bottom_up = bottom_up;
     m_u = m_u > m_f ? m_u - m_f : 0;
     n_in = c[0];
     m_f = c[1];
