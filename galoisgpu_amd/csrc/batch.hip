@@ -83,12 +83,56 @@ struct MArgs {
 };
 
 // Per-round, per-lane view of the schedule used by the edge loop.
+// Pushes are staged per wavefront in LDS, kPushBuf items per pile, and
+// written out with one queue-tail atomic per flush: one atomic per wave_push
+// on a tail that every wavefront of the grid shares serialises at the
+// memory-side atomic unit (~12 ns each on gfx950).
+#ifndef LSG_PUSH_BUF
+#define LSG_PUSH_BUF 256
+#endif
+constexpr uint32_t kPushBuf = LSG_PUSH_BUF;
+
 struct LaneSched {
   uint32_t thr;   // near/far threshold of this lane's source
   bool bucket;    // per-source thresholds (LSG_FAR_BUCKET phase)
   uint32_t fmin;  // smallest distance this lane pushed to the far pile this round
   uint32_t thr32; // threshold of this lane as u32: a candidate nd goes near iff nd <= thr32
+  uint32_t *pb;   // this wavefront's LDS push buffers: [0, kPushBuf) near, [kPushBuf, 2 kPushBuf) far
+  uint32_t pbn[2];  // items staged per pile (wave-uniform)
 };
+
+// Writes pile `which`'s staged items to its queue (one atomic for all).
+__device__ __forceinline__ void push_flush(const MArgs &a, const Sched &s, LaneSched &ls, int which) {
+  const uint32_t n = ls.pbn[which];
+  if (n == 0) return;
+  const uint32_t qi = which ? s.far_q : s.near_q;
+  uint32_t base = 0;
+  if (lane_id() == 0) base = atomic_add(&a.ctrl->qcount[qi], n);
+  base = readlane(base, 0);
+  const uint32_t *buf = ls.pb + which * kPushBuf;
+  for (uint32_t i = lane_id(); i < n; i += kWave) {
+    const uint32_t pos = base + i;
+    if (pos < a.qcap) st_coh(a.q[qi] + pos, buf[i]);
+    else st_coh(&a.ctrl->abort, kAbortCapacity);
+  }
+  ls.pbn[which] = 0;
+}
+
+// Pushes `item` from every lane with `pred` onto pile `which` (0 near, 1 far).
+__device__ __forceinline__ void pile_push(const MArgs &a, const Sched &s, LaneSched &ls, int which, bool pred,
+                                          uint32_t item) {
+#if LSG_PUSH_BUF
+  const uint64_t m = __ballot(pred);
+  if (m == 0) return;
+  const uint32_t n = (uint32_t)__builtin_popcountll(m);
+  if (ls.pbn[which] + n > kPushBuf) push_flush(a, s, ls, which);
+  if (pred) ls.pb[which * kPushBuf + ls.pbn[which] + lanes_below(m)] = item;
+  ls.pbn[which] += n;
+#else
+  const uint32_t qi = which ? s.far_q : s.near_q;
+  wave_push(pred, item, a.q[qi], &a.ctrl->qcount[qi], a.qcap, &a.ctrl->abort);
+#endif
+}
 
 // Counters the leader turns into trace records and scheduler features.
 struct MultiCounters {
@@ -169,8 +213,8 @@ __device__ __forceinline__ void relax_batch(const MArgs &a, const Sched &s, Lane
   // finds the mask empty queues it.
   bool pn = my_nb != 0 && atomic_or64(a.nmask[s.round & 1] + ev, my_nb) == 0 && near_admit(a, s, ev);
   bool pf = my_fb != 0 && atomic_or64(a.fmask[s.phase & 1] + ev, my_fb) == 0;
-  wave_push(pn, ev, a.q[s.near_q], &a.ctrl->qcount[s.near_q], a.qcap, &a.ctrl->abort);
-  wave_push(pf, ev, a.q[s.far_q], &a.ctrl->qcount[s.far_q], a.qcap, &a.ctrl->abort);
+  pile_push(a, s, ls, 0, pn, ev);
+  pile_push(a, s, ls, 1, pf, ev);
 }
 
 // ---- item-major relaxation ------------------------------------------------
@@ -184,6 +228,9 @@ __device__ __forceinline__ void relax_batch(const MArgs &a, const Sched &s, Lane
 // 16 wide 34.2 (pipelined or not; pipelined it spills).  Improvements are
 // collected in a per-wavefront list and the destinations' masks are updated
 // 64 at a time.
+#ifndef LSG_MULTI_STATIC
+#define LSG_MULTI_STATIC 0
+#endif
 #ifndef LSG_GROUP
 #define LSG_GROUP 8
 #endif
@@ -211,14 +258,14 @@ struct Pending {
 // Sets the pending destinations' mask bits; a destination whose mask was
 // empty joins the pile's queue (two entries for one destination: only the
 // lane whose OR finds the mask empty pushes it).
-__device__ __forceinline__ void flush_pending(const MArgs &a, const Sched &s, Pending &p) {
+__device__ __forceinline__ void flush_pending(const MArgs &a, const Sched &s, LaneSched &ls, Pending &p) {
   if (p.n == 0) return;
   const bool act = lane_id() < p.n;
   const uint64_t nb = ((uint64_t)p.nb_hi << 32) | p.nb_lo, fb = ((uint64_t)p.fb_hi << 32) | p.fb_lo;
   const bool pn = act && nb != 0 && atomic_or64(a.nmask[s.round & 1] + p.v, nb) == 0 && near_admit(a, s, p.v);
   const bool pf = act && fb != 0 && atomic_or64(a.fmask[s.phase & 1] + p.v, fb) == 0;
-  wave_push(pn, p.v, a.q[s.near_q], &a.ctrl->qcount[s.near_q], a.qcap, &a.ctrl->abort);
-  wave_push(pf, p.v, a.q[s.far_q], &a.ctrl->qcount[s.far_q], a.qcap, &a.ctrl->abort);
+  pile_push(a, s, ls, 0, pn, p.v);
+  pile_push(a, s, ls, 1, pf, p.v);
   p.n = 0;
 }
 
@@ -310,7 +357,7 @@ __device__ __forceinline__ void relax_edges(const MArgs &a, const Sched &s, Lane
     const uint32_t wn = more ? min(en, hi - kGroup) : w0;
     const U32xG vvn = scalar_window(a.edge_dst, wn), wwn = scalar_window(a.edge_data, wn);
     if (have_prev) window_results(a, ls, prev, p, improved);
-    if (p.n > (uint32_t)(kWave - kGroup)) flush_pending(a, s, p);
+    if (p.n > (uint32_t)(kWave - kGroup)) flush_pending(a, s, ls, p);
     window_atomics(a, vv, col, cur);
     prev = cur;
     have_prev = true;
@@ -393,7 +440,7 @@ __device__ __forceinline__ void process_items(const MArgs &a, const Sched &s, La
         const uint32_t uk = readlane(u, k);
         bool push = false;
         if (lane == 0) push = atomic_or64(a.fmask[s.phase & 1] + uk, dm) == 0;
-        wave_push(push, uk, a.q[s.far_q], &a.ctrl->qcount[s.far_q], a.qcap, &a.ctrl->abort);
+        pile_push(a, s, ls, 1, push, uk);
         if (defer) ls.fmin = min(ls.fmin, du[k]);
         act = act && !defer;
       }
@@ -598,6 +645,9 @@ void multi_kernel(MArgs a, MultiCounters *tot) {
   __shared__ uint32_t quit;
   __shared__ unsigned long long blk[3];
   __shared__ DistSlots dls[BS / kWave];
+#if LSG_PUSH_BUF
+  __shared__ uint32_t pushbuf[BS / kWave][2 * kPushBuf];
+#endif
   Ctrl *c = a.ctrl;
   const uint32_t nb = gridDim.x;
   if (!residency_handshake(c, nb, a.residency_limit)) return;
@@ -621,6 +671,10 @@ void multi_kernel(MArgs a, MultiCounters *tot) {
     ls.thr = ls.bucket && lane_id() < a.S ? ld_coh(a.thr + lane_id()) : 0u;
     ls.thr32 = ls.bucket ? ls.thr : (uint32_t)min(s.curdelta(), (int64_t)UINT32_MAX);
     ls.fmin = kInf;
+#if LSG_PUSH_BUF
+    ls.pb = &pushbuf[threadIdx.x / kWave][0];
+#endif
+    ls.pbn[0] = ls.pbn[1] = 0;
     Pending p;
     p.n = 0;
     p.v = p.nb_lo = p.nb_hi = p.fb_lo = p.fb_hi = 0;
@@ -628,6 +682,10 @@ void multi_kernel(MArgs a, MultiCounters *tot) {
     // Static first pass (ticket = wavefront id), then dynamic tickets; the
     // next ticket is fetched while the current one is processed.
     const uint32_t nchunks = (s.in_count + kItems - 1) / kItems;
+#if LSG_MULTI_STATIC
+    for (uint32_t ck = wave; ck < nchunks; ck += nwaves) process_items(a, s, ls, ck * kItems, dl, mc, p);
+    for (uint32_t ck = wave; ck < s.heavy_count; ck += nwaves) process_chunk(a, s, ls, ck, mc, p);
+#else
     for (uint32_t ck = wave; ck < nchunks;) {
       uint32_t t = 0;
       if (lane_id() == 0) t = atomic_add(&c->ticket, 1u);
@@ -640,7 +698,12 @@ void multi_kernel(MArgs a, MultiCounters *tot) {
       process_chunk(a, s, ls, ck, mc, p);
       ck = nwaves + readlane(t, 0);
     }
-    flush_pending(a, s, p);
+#endif
+    flush_pending(a, s, ls, p);
+#if LSG_PUSH_BUF
+    push_flush(a, s, ls, 0);
+    push_flush(a, s, ls, 1);
+#endif
     // this wavefront's smallest far push per source -> phase minimum
     if (__ballot(ls.fmin != kInf) && ls.fmin != kInf) atomic_min(a.fmin + (s.phase & 1) * 64 + lane_id(), ls.fmin);
     if (lane_id() == 0 && (mc.lane_items | mc.edges)) {

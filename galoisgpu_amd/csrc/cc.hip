@@ -77,7 +77,14 @@ __global__ void cc_hook_kernel(const uint32_t *row_start, const uint32_t *edge_d
       ru = find_root(label, u);
     }
   }
-  if (__ballot(changed) && lane_id() == 0) st_coh(ctl, 1u);
+  // one store per block that changed anything: a store per wavefront to the
+  // same word serialises like a same-address atomic
+  __shared__ uint32_t s_changed;
+  if (threadIdx.x == 0) s_changed = 0;
+  __syncthreads();
+  if (__ballot(changed) && lane_id() == 0) s_changed = 1;
+  __syncthreads();
+  if (threadIdx.x == 0 && s_changed) st_coh(ctl, 1u);
 }
 
 // Pointer jumping: every vertex's parent becomes its root.
@@ -89,12 +96,17 @@ __global__ void cc_jump_kernel(uint32_t *label, uint32_t V) {
 }
 
 __global__ void cc_count_kernel(const uint32_t *label, uint32_t V, uint32_t *ctl) {
+  __shared__ uint32_t s_n;
+  if (threadIdx.x == 0) s_n = 0;
+  __syncthreads();
   uint32_t n = 0;
   for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < V; v += (size_t)gridDim.x * blockDim.x)
     n += label[v] == v;
 #pragma unroll
   for (int d = kWave / 2; d >= 1; d >>= 1) n += __shfl_xor(n, d);
-  if (lane_id() == 0 && n) atomic_add(ctl + 1, n);
+  if (lane_id() == 0 && n) atomicAdd(&s_n, n);
+  __syncthreads();
+  if (threadIdx.x == 0 && s_n) atomic_add(ctl + 1, s_n);
 }
 
 __global__ void cc_clear_kernel(uint32_t *ctl) { ctl[0] = 0; }

@@ -72,11 +72,18 @@ __device__ __forceinline__ void pr_store(const PrArgs &a, uint32_t v, float sum,
   dmax = fmaxf(dmax, fabsf(pn - a.p_cur[v]) / pn);  // pn >= 1 - alpha > 0
 }
 
+// The block's largest change goes to maxdiff with one atomic per block (one
+// per wavefront on the same word serialises at the memory-side atomic unit).
 __device__ __forceinline__ void pr_reduce_max(const PrArgs &a, float dmax) {
+  __shared__ uint32_t s_max;
+  if (threadIdx.x == 0) s_max = 0;
+  __syncthreads();
 #pragma unroll
   for (int d = kWave / 2; d >= 1; d >>= 1) dmax = fmaxf(dmax, __shfl_xor(dmax, d));
-  if (lane_id() == 0 && dmax > 0.0f)
-    __hip_atomic_fetch_max(a.maxdiff, __float_as_uint(dmax), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (lane_id() == 0 && dmax > 0.0f) atomicMax(&s_max, __float_as_uint(dmax));
+  __syncthreads();
+  if (threadIdx.x == 0 && s_max)
+    __hip_atomic_fetch_max(a.maxdiff, s_max, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // One lane per vertex of in-degree < kWaveDeg.
@@ -86,8 +93,20 @@ __global__ void pr_light_kernel(PrArgs a) {
   for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < a.nnodes; v += nt) {
     const uint32_t e0 = a.in_row[v], e1 = a.in_row[v + 1];
     if (e1 - e0 >= kWaveDeg) continue;
+    // four in-edges per step, summed in edge order
     float sum = 0.0f;
-    for (uint32_t e = e0; e < e1; ++e) sum += a.c_cur[a.in_src[e]];
+    uint32_t e = e0;
+    for (; e + 4 <= e1; e += 4) {
+      uint32_t u[4];
+      float c[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) u[k] = a.in_src[e + k];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) c[k] = a.c_cur[u[k]];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) sum += c[k];
+    }
+    for (; e < e1; ++e) sum += a.c_cur[a.in_src[e]];
     pr_store(a, (uint32_t)v, sum, dmax);
   }
   pr_reduce_max(a, dmax);
@@ -102,8 +121,18 @@ __global__ void pr_heavy_kernel(PrArgs a) {
   for (size_t i = (blockIdx.x * (size_t)blockDim.x + threadIdx.x) / kWave; i < a.nheavy; i += waves) {
     const uint32_t v = a.heavy[i];
     const uint32_t e0 = a.in_row[v], e1 = a.in_row[v + 1];
+    // lane l sums in-edges e0 + l, e0 + l + 64, ... in that order, four per step
     float sum = 0.0f;
-    for (uint32_t e = e0 + lane; e < e1; e += kWave) sum += a.c_cur[a.in_src[e]];
+    for (uint32_t e = e0 + lane; e < e1; e += 4 * kWave) {
+      uint32_t u[4];
+      float c[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) u[k] = e + k * kWave < e1 ? a.in_src[e + k * kWave] : 0u;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) c[k] = e + k * kWave < e1 ? a.c_cur[u[k]] : 0.0f;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) sum += c[k];
+    }
 #pragma unroll
     for (int d = kWave / 2; d >= 1; d >>= 1) sum += __shfl_xor(sum, d);
     if (lane == 0) pr_store(a, v, sum, dmax);

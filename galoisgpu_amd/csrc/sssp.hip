@@ -69,6 +69,72 @@ struct KArgs {
   uint64_t nedges;
 };
 
+// ---- pile pushes -----------------------------------------------------------
+
+// Pushes are staged per wavefront in LDS, kStage items per pile, and written
+// out with one queue-tail atomic per flush (and at the end of every round):
+// one atomic per wave_push on a tail that every wavefront of the grid shares
+// serialises at the memory-side atomic unit (~12 ns each on gfx950).
+#ifndef LSG_PLAIN_PUSH_BUF
+#define LSG_PLAIN_PUSH_BUF 256
+#endif
+constexpr uint32_t kStage = LSG_PLAIN_PUSH_BUF > 0 ? LSG_PLAIN_PUSH_BUF : 1;
+
+template <int BS>
+struct PushStage {
+  uint32_t item[BS / kWave][2][kStage];  // [wave][near, far]
+  uint32_t n[BS / kWave][2];
+};
+
+// Writes this wavefront's staged items of pile `which` (0 near, 1 far) to the queue.
+template <int BS>
+__device__ __forceinline__ void stage_flush(const KArgs &a, const Sched &s, PushStage<BS> &ps, int which) {
+  const uint32_t w = threadIdx.x / kWave;
+  const uint32_t n = __builtin_amdgcn_readfirstlane(ps.n[w][which]);
+  if (n == 0) return;
+  const uint32_t qi = which ? s.far_q : s.near_q;
+  uint32_t base = 0;
+  if (lane_id() == 0) base = atomic_add(&a.ctrl->qcount[qi], n);
+  base = readlane(base, 0);
+  for (uint32_t i = lane_id(); i < n; i += kWave) {
+    const uint32_t pos = base + i;
+    if (pos < a.qcap) st_coh(a.q[qi] + pos, ps.item[w][which][i]);
+    else st_coh(&a.ctrl->abort, kAbortCapacity);
+  }
+  ps.n[w][which] = 0;
+}
+
+// Pushes `item` from every lane with `pred` onto pile `which`.
+template <int BS>
+__device__ __forceinline__ void stage_push(const KArgs &a, const Sched &s, PushStage<BS> &ps, int which, bool pred,
+                                           uint32_t item) {
+#if LSG_PLAIN_PUSH_BUF
+  const uint64_t m = __ballot(pred);
+  if (m == 0) return;
+  const uint32_t w = threadIdx.x / kWave;
+  const uint32_t c = (uint32_t)__builtin_popcountll(m);
+  uint32_t n = __builtin_amdgcn_readfirstlane(ps.n[w][which]);
+  if (n + c > kStage) {
+    stage_flush<BS>(a, s, ps, which);
+    n = 0;
+  }
+  if (pred) ps.item[w][which][n + lanes_below(m)] = item;
+  ps.n[w][which] = n + c;
+#else
+  const uint32_t qi = which ? s.far_q : s.near_q;
+  wave_push(pred, item, a.q[qi], &a.ctrl->qcount[qi], a.qcap, &a.ctrl->abort);
+#endif
+}
+
+// End of a round: every wavefront writes out what it staged.
+template <int BS>
+__device__ __forceinline__ void stage_drain(const KArgs &a, const Sched &s, PushStage<BS> &ps) {
+#if LSG_PLAIN_PUSH_BUF
+  stage_flush<BS>(a, s, ps, 0);
+  stage_flush<BS>(a, s, ps, 1);
+#endif
+}
+
 // ---- per-edge operators ---------------------------------------------------
 
 // Relax u->v with candidate nd = d[u] + w.  Same test and atomicMin as
@@ -77,8 +143,9 @@ struct KArgs {
 // near/far split at curdelta (:134).  Arithmetic is u32: d[u] < 2^31 and
 // w <= INT_MAX never wrap, and a candidate >= INT_MAX never beats INF.
 struct SsspRelax {
-  __device__ __forceinline__ static void edge(const KArgs &a, const Sched &s, uint32_t v, uint32_t nd,
-                                              bool active) {
+  template <int BS>
+  __device__ __forceinline__ static void edge(const KArgs &a, const Sched &s, uint32_t v, uint32_t nd, bool active,
+                                              PushStage<BS> &ps) {
     // The pre-checks read through the cache: a stale d[v] is only ever larger
     // than the current one (costing at most a redundant atomicMin, whose
     // return value is authoritative), and a stale stamp is only ever older.
@@ -101,8 +168,8 @@ struct SsspRelax {
     // racing improvers, the cached pre-check skips it when already stamped.
     const bool push = imp && seen != stamp && atomic_exch(stamps + v, stamp) != stamp;
     const bool to_near = push && near, to_far = push && !near;
-    wave_push(to_near, v, a.q[s.near_q], &a.ctrl->qcount[s.near_q], a.qcap, &a.ctrl->abort);
-    wave_push(to_far, v, a.q[s.far_q], &a.ctrl->qcount[s.far_q], a.qcap, &a.ctrl->abort);
+    stage_push<BS>(a, s, ps, 0, to_near, v);
+    stage_push<BS>(a, s, ps, 1, to_far, v);
   }
 };
 
@@ -112,8 +179,9 @@ struct SsspRelax {
 // one round after their vertex is popped) still ends at its hop count; every
 // improvement is pushed to the next level's pile, once per round.
 struct BfsVisit {
-  __device__ __forceinline__ static void edge(const KArgs &a, const Sched &s, uint32_t v, uint32_t nd,
-                                              bool active) {
+  template <int BS>
+  __device__ __forceinline__ static void edge(const KArgs &a, const Sched &s, uint32_t v, uint32_t nd, bool active,
+                                              PushStage<BS> &ps) {
     bool imp = false;
     if (active) {
       uint32_t dv = a.dist[v];
@@ -123,19 +191,20 @@ struct BfsVisit {
       uint32_t stamp = s.round + 1;
       imp = a.near_stamp[v] != stamp && atomic_exch(a.near_stamp + v, stamp) != stamp;
     }
-    wave_push(imp, v, a.q[s.near_q], &a.ctrl->qcount[s.near_q], a.qcap, &a.ctrl->abort);
+    stage_push<BS>(a, s, ps, 0, imp, v);
   }
 };
 
-template <Algo A>
-__device__ __forceinline__ void relax_edge(const KArgs &a, const Sched &s, uint32_t e, uint32_t du, bool active) {
+template <Algo A, int BS>
+__device__ __forceinline__ void relax_edge(const KArgs &a, const Sched &s, uint32_t e, uint32_t du, bool active,
+                                           PushStage<BS> &ps) {
   uint32_t v = 0, nd = 0;
   if (active) {
     v = a.edge_dst[e];
     nd = A == Algo::SSSP ? du + a.edge_data[e] : du + 1u;
   }
-  if (A == Algo::SSSP) SsspRelax::edge(a, s, v, nd, active);
-  else BfsVisit::edge(a, s, v, nd, active);
+  if (A == Algo::SSSP) SsspRelax::edge<BS>(a, s, v, nd, active, ps);
+  else BfsVisit::edge<BS>(a, s, v, nd, active, ps);
 }
 
 // ---- block-level helpers ---------------------------------------------------
@@ -179,6 +248,7 @@ struct TileShared {
   uint32_t du[BS];
   uint32_t wsum[BS / 64 + 1];
   unsigned long long edges;  // edges expanded by this block in the current round
+  PushStage<BS> ps;
 };
 
 // One tile of the in-queue: items [t*ti, t*ti+ti), ti <= BS items, one per
@@ -209,7 +279,7 @@ __device__ void process_tile(const KArgs &a, const Sched &s, uint32_t t, uint32_
       defer = a.far_stamp[u] != stamp && atomic_exch(a.far_stamp + u, stamp) != stamp;
       valid = false;
     }
-    wave_push(defer, u, a.q[s.far_q], &a.ctrl->qcount[s.far_q], a.qcap, &a.ctrl->abort);
+    stage_push<BS>(a, s, sh.ps, 1, defer, u);
   }
   if (valid) deg = re - rs;
   // Heavy vertices: split into ceil(deg / heavy_deg) chunks, processed by
@@ -260,7 +330,7 @@ __device__ void process_tile(const KArgs &a, const Sched &s, uint32_t t, uint32_
         ee = sh.start[lo] + (e - sh.offs[lo]);
         d0 = sh.du[lo];
       }
-      relax_edge<A>(a, s, ee, d0, act);
+      relax_edge<A, BS>(a, s, ee, d0, act, sh.ps);
     }
     __syncthreads();
   } else {
@@ -271,7 +341,7 @@ __device__ void process_tile(const KArgs &a, const Sched &s, uint32_t t, uint32_
     uint32_t mx = my;
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, off));
-    for (uint32_t j = 0; j < mx; ++j) relax_edge<A>(a, s, rs + j, du, j < my);
+    for (uint32_t j = 0; j < mx; ++j) relax_edge<A, BS>(a, s, rs + j, du, j < my, sh.ps);
     uint64_t med = __ballot(!small && deg > 0);
     while (med) {
       uint32_t l = (uint32_t)__builtin_ctzll(med);
@@ -279,7 +349,7 @@ __device__ void process_tile(const KArgs &a, const Sched &s, uint32_t t, uint32_
       uint32_t ws = readlane(rs, l), wd = readlane(deg, l), wu = readlane(du, l);
       for (uint32_t j = 0; j < wd; j += kWave) {
         uint32_t jj = j + lane_id();
-        relax_edge<A>(a, s, ws + jj, wu, jj < wd);
+        relax_edge<A, BS>(a, s, ws + jj, wu, jj < wd, sh.ps);
       }
     }
   }
@@ -300,7 +370,7 @@ __device__ void process_heavy(const KArgs &a, const Sched &s, uint32_t c, TileSh
   if ((a.sched || a.feat) && threadIdx.x == 0) atomicAdd(&sh.edges, (unsigned long long)(hi - lo));
   for (uint32_t b = lo; b < hi; b += BS) {
     uint32_t e = b + threadIdx.x;
-    relax_edge<A>(a, s, e, du, e < hi);
+    relax_edge<A, BS>(a, s, e, du, e < hi, sh.ps);
   }
 }
 
@@ -453,6 +523,7 @@ __device__ void local_rounds(const KArgs &a, TileShared<BS> &sh) {
       process_tile<A, BS>(a, s, t, BS, sh);
       __syncthreads();
     }
+    stage_drain<BS>(a, s, sh.ps);
     if (threadIdx.x == 0 && (a.sched || a.feat) && sh.edges)
       __hip_atomic_fetch_add(&c->stat_edges, (uint64_t)sh.edges, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     // this block's pushes and counts are complete before the leader step reads them
@@ -472,6 +543,7 @@ __global__ __launch_bounds__(BS) void persistent_kernel(KArgs a) {
   Ctrl *c = a.ctrl;
   const uint32_t nb = gridDim.x;
   if (!residency_handshake(c, nb, a.residency_limit)) return;
+  if (threadIdx.x < 2 * (BS / kWave)) (&sh.ps.n[0][0])[threadIdx.x] = 0;
   for (;;) {
     if (threadIdx.x == 0) {
       Sched s;
@@ -500,6 +572,7 @@ __global__ __launch_bounds__(BS) void persistent_kernel(KArgs a) {
       __syncthreads();
       t = tk;
     }
+    stage_drain<BS>(a, s, sh.ps);
     if (a.sched || a.feat) {
       __syncthreads();
       if (threadIdx.x == 0 && sh.edges)
