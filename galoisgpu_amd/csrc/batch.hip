@@ -231,6 +231,10 @@ __device__ __forceinline__ void relax_batch(const MArgs &a, const Sched &s, Lane
 #ifndef LSG_MULTI_STATIC
 #define LSG_MULTI_STATIC 0
 #endif
+#ifndef LSG_TICKET_SPAN
+#define LSG_TICKET_SPAN 2
+#endif
+constexpr uint32_t kSpan = LSG_TICKET_SPAN;
 #ifndef LSG_GROUP
 #define LSG_GROUP 8
 #endif
@@ -686,17 +690,31 @@ void multi_kernel(MArgs a, MultiCounters *tot) {
     for (uint32_t ck = wave; ck < nchunks; ck += nwaves) process_items(a, s, ls, ck * kItems, dl, mc, p);
     for (uint32_t ck = wave; ck < s.heavy_count; ck += nwaves) process_chunk(a, s, ls, ck, mc, p);
 #else
-    for (uint32_t ck = wave; ck < nchunks;) {
+    // A ticket covers kSpan consecutive work units; the next ticket is taken
+    // when the last unit of the current range starts.
+    for (uint32_t ck = wave, cend = wave + 1; ck < nchunks;) {
+      const bool last = ck + 1 >= cend;
       uint32_t t = 0;
-      if (lane_id() == 0) t = atomic_add(&c->ticket, 1u);
+      if (last && lane_id() == 0) t = atomic_add(&c->ticket, 1u);
       process_items(a, s, ls, ck * kItems, dl, mc, p);
-      ck = nwaves + readlane(t, 0);
+      if (last) {
+        ck = nwaves + readlane(t, 0) * kSpan;
+        cend = ck + kSpan;
+      } else {
+        ++ck;
+      }
     }
-    for (uint32_t ck = wave; ck < s.heavy_count;) {
+    for (uint32_t ck = wave, cend = wave + 1; ck < s.heavy_count;) {
+      const bool last = ck + 1 >= cend;
       uint32_t t = 0;
-      if (lane_id() == 0) t = atomic_add(&c->ticket2, 1u);
+      if (last && lane_id() == 0) t = atomic_add(&c->ticket2, 1u);
       process_chunk(a, s, ls, ck, mc, p);
-      ck = nwaves + readlane(t, 0);
+      if (last) {
+        ck = nwaves + readlane(t, 0) * kSpan;
+        cend = ck + kSpan;
+      } else {
+        ++ck;
+      }
     }
 #endif
     flush_pending(a, s, ls, p);
