@@ -3,20 +3,20 @@
 // The levels are the reference's (apps/bfs/kernel.cu:20-32,312-334,380-393):
 // level[src] = 0, first hop LEVEL 1, INF = INT_MAX for unreachable vertices.
 // BFS levels are unique, so how a level is discovered does not change them.
-// Each level is either
-//   top-down:  the frontier queue's out-edges are flattened across the grid
-//              (device prefix sum of degrees, one edge per thread) and an
+// All levels run in one persistent launch with a grid barrier between them.
+// Every level marks the vertices it discovers in a bitmap (three bitmaps
+// rotate by level mod 3) and is either
+//   top-down:  the previous level's vertices -- read from its bitmap, the
+//              heavy ones as chunks of edges queued when they were found --
+//              have their out-edges flattened over each wavefront, and an
 //              unvisited destination is claimed with a compare-and-swap on
 //              its level;
 //   bottom-up: every unvisited vertex scans its in-edges and stops at the
-//              first parent whose level is L-1.
-// The host picks the direction per level from the frontier's size and edge
-// count (Beamer, Asanovic and Patterson's rule: bottom-up once the frontier's
-// edges exceed 1/alpha of the unexplored edges, top-down again once the
-// frontier holds fewer than 1/beta of the vertices).  Every level marks the
-// vertices it discovers in a bitmap; a top-down level also queues them, and
-// a top-down level after a bottom-up one compacts that level's bitmap into
-// its queue first.
+//              first parent whose bit is set in the previous level's bitmap.
+// The barrier leader picks the direction of the next level from the
+// frontier's size and edge count (Beamer, Asanovic and Patterson's rule:
+// bottom-up once the frontier's edges exceed 1/alpha of the unexplored edges,
+// top-down again once the frontier holds fewer than 1/beta of the vertices).
 
 #include <hip/hip_runtime.h>
 
@@ -143,127 +143,137 @@ __global__ void scatter_in_kernel(const uint32_t *row_start, const uint32_t *edg
   }
 }
 
-// ---- the two BFS steps ---------------------------------------------------------
+constexpr uint32_t kBuRun = 4;  // in-edges a bottom-up lane tests per step
 
-struct DoArgs {
+// ---- the persistent traversal -------------------------------------------------
+//
+// All levels run in one co-resident launch; the barrier leader applies the
+// direction rule between levels, so no level waits for the host.  Words that
+// other blocks wrote earlier in the launch (levels, bitmaps, chunk queues,
+// per-block counts) are read and written with agent-scope atomics or coherent
+// loads and stores: the XCDs' L2 caches are not coherent with each other.
+// The graph arrays are read-only and read through the caches.
+
+constexpr uint32_t kTdHeavy = 256;  // out-degree from which a frontier vertex is spread as chunks
+constexpr uint32_t kTdChunk = 256;  // edges per chunk
+
+struct PdoArgs {
   const uint32_t *row_start, *edge_dst;  // forward CSR
   const uint32_t *in_row, *in_src;       // reverse CSR
   uint32_t *level;
-  const uint32_t *q_in;
-  uint32_t *q_out;
-  uint32_t *ctr;  // [2][2]: (pushed, their out-edge count) by level parity
-  uint32_t *fb[3];  // frontier bitmaps: fb[L % 3] has bit v set iff level[v] == L
-  uint32_t nnodes, n_in, L;
+  uint32_t *fb[3];       // fb[L % 3] bit v: level[v] == L
+  uint64_t *hq[2];       // chunks (k << 32 | u) of the heavy vertices found at level L, in hq[L & 1]
+  uint32_t *partial;     // [block][2]: vertices found by the block this level, their out-edges
+  Ctrl *ctrl;            // hcount[L & 1]: chunks queued at level L
+  uint32_t nnodes, words, hcap, direction;
+  uint64_t spin_limit, residency_limit;
 };
 
-// Pushes the vertices with `found` (wave-aggregated), sets their bits in the
-// level's frontier bitmap and adds their out-degree to the level's edge count.
-__device__ __forceinline__ void do_push(const DoArgs &a, bool found, uint32_t v) {
-  const uint64_t m = __ballot(found);
-  if (m == 0) return;
-  if (found) atomicOr(a.fb[a.L % 3] + (v >> 5), 1u << (v & 31));
-  uint32_t *c = a.ctr + 2 * (a.L & 1);
-  const uint32_t leader = (uint32_t)__builtin_ctzll(m);
-  uint32_t deg = found ? a.row_start[v + 1] - a.row_start[v] : 0u;
-#pragma unroll
-  for (int d = kWave / 2; d >= 1; d >>= 1) deg += __shfl_xor(deg, d);
-  uint32_t base = 0;
-  if (lane_id() == leader) {
-    base = atomic_add(c, (uint32_t)__builtin_popcountll(m));
-    atomic_add(c + 1, deg);
-  }
-  base = readlane(base, leader);
-  if (found) a.q_out[base + lanes_below(m)] = v;
-}
+// Per-block tallies of one level.
+struct PdoShared {
+  uint32_t cnt, deg;
+};
 
-// Run by every level's kernel: the next level's counters start at zero, and
-// the bitmap the next level will fill (last used two levels ago) is cleared.
-__device__ __forceinline__ void clear_next(const DoArgs &a) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    uint32_t *n = a.ctr + 2 * ((a.L + 1) & 1);
-    n[0] = 0;
-    n[1] = 0;
+// Records vertices discovered at level L: counts, out-edges, and chunks for
+// the heavy ones (used if the next level is top-down).
+__device__ __forceinline__ void pdo_found(const PdoArgs &a, uint32_t L, bool found, uint32_t v, uint32_t &cnt,
+                                          uint32_t &deg) {
+  uint32_t d = 0;
+  if (found) {
+    d = a.row_start[v + 1] - a.row_start[v];
+    cnt += 1;
+    deg += d;
   }
-  uint32_t *nb = a.fb[(a.L + 1) % 3];
-  const uint32_t words = (a.nnodes + 31) / 32;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < words; i += (size_t)gridDim.x * blockDim.x)
-    nb[i] = 0;
-}
-
-// Out-degrees of the frontier, the input of the top-down prefix sum.
-__global__ void frontier_deg_kernel(const uint32_t *q, uint32_t n, const uint32_t *row_start, uint32_t *deg) {
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-    const uint32_t u = q[i];
-    deg[i] = row_start[u + 1] - row_start[u];
+  const bool heavy = found && d >= kTdHeavy;
+  if (__ballot(heavy) == 0) return;
+  const uint32_t nch = heavy ? (d + kTdChunk - 1) / kTdChunk : 0u;
+  const uint32_t base = wave_reserve(nch, &a.ctrl->hcount[L & 1]);
+  for (uint32_t k = 0; k < nch; ++k) {
+    if (base + k < a.hcap) st_coh64(a.hq[L & 1] + base + k, ((uint64_t)k << 32) | v);
+    else st_coh(&a.ctrl->abort, kAbortCapacity);
   }
 }
 
-// Top-down level L over the frontier's out-edges flattened across the whole
-// grid: pre[i] is the first flattened edge of frontier vertex i (exclusive
-// prefix sum of degrees, pre[n_in] = total), so one vertex's edges -- the
-// source hub's at level 1 -- spread over every block.  Each thread finds
-// the owner of its edge by binary search and claims an unvisited destination
-// with a compare-and-swap on its level.
-__global__ __launch_bounds__(kDoBlock) void top_down_kernel(DoArgs a, const uint32_t *pre) {
-  clear_next(a);
-  const uint32_t total = pre[a.n_in];
-  const size_t nt = (size_t)gridDim.x * blockDim.x;
-  // whole wavefronts iterate together so that the push's ballot sees them all
-  const size_t fend = (total + (size_t)kWave - 1) / kWave * kWave;
-  for (size_t f = blockIdx.x * (size_t)blockDim.x + threadIdx.x; f < fend; f += nt) {
-    bool found = false;
-    uint32_t v = 0;
-    if (f < total) {
-      uint32_t lo = 0, hi = a.n_in;  // last i with pre[i] <= f
-      while (hi - lo > 1) {
-        const uint32_t mid = lo + (hi - lo) / 2;
-        if (pre[mid] <= f) lo = mid;
-        else hi = mid;
+// Top-down visit of v at level L: claims it with a compare-and-swap.
+__device__ __forceinline__ void pdo_visit(const PdoArgs &a, uint32_t L, bool valid, uint32_t v, uint32_t &cnt,
+                                          uint32_t &deg) {
+  bool found = false;
+  if (valid && ld_coh(a.level + v) == kInf) {
+    uint32_t expect = kInf;
+    found = __hip_atomic_compare_exchange_strong(a.level + v, &expect, L, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+  }
+  if (found)
+    __hip_atomic_fetch_or(a.fb[L % 3] + (v >> 5), 1u << (v & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  pdo_found(a, L, found, v, cnt, deg);
+}
+
+// Top-down level L.  The heavy frontier vertices come as chunks of kTdChunk
+// edges, one wavefront per chunk; the light ones are read from the previous
+// level's bitmap, 64 words per wavefront pass: each step takes one vertex per
+// lane, and their edges are flattened over the wavefront (a prefix sum of the
+// degrees, each lane finding its edge's owner by binary search over the lanes).
+__device__ __forceinline__ void pdo_top_down(const PdoArgs &a, const Sched &s, uint32_t gwave, uint32_t nwaves,
+                                             uint32_t &cnt, uint32_t &deg) {
+  const uint32_t L = s.round, lane = lane_id();
+  const uint64_t *hq = a.hq[(L - 1) & 1];
+  for (uint32_t c = gwave; c < s.heavy_count; c += nwaves) {
+    const uint64_t ent = ld_coh64(hq + c);
+    const uint32_t u = (uint32_t)ent, k = (uint32_t)(ent >> 32);
+    const uint32_t rs = a.row_start[u], re = a.row_start[u + 1];
+    const uint32_t lo = rs + k * kTdChunk, hi = min(re, lo + kTdChunk);
+    for (uint32_t e0 = lo; e0 < hi; e0 += kWave) {
+      const uint32_t e = e0 + lane;
+      pdo_visit(a, L, e < hi, e < hi ? a.edge_dst[e] : 0u, cnt, deg);
+    }
+  }
+  const uint32_t *fr = a.fb[(L - 1) % 3];
+  for (size_t w0 = (size_t)gwave * kWave; w0 < a.words; w0 += (size_t)nwaves * kWave) {
+    const size_t wi = w0 + lane;
+    uint32_t bits = wi < a.words ? ld_coh(fr + wi) : 0u;
+    while (__ballot(bits != 0)) {
+      uint32_t rs = 0, dg = 0;
+      if (bits) {
+        const uint32_t u = (uint32_t)wi * 32 + (uint32_t)__builtin_ctz(bits);
+        bits &= bits - 1;
+        rs = a.row_start[u];
+        dg = a.row_start[u + 1] - rs;
+        if (dg >= kTdHeavy) dg = 0;  // expanded through its chunks
       }
-      v = a.edge_dst[a.row_start[a.q_in[lo]] + ((uint32_t)f - pre[lo])];
-      if (a.level[v] == kInf) {
-        uint32_t expect = kInf;
-        found = __hip_atomic_compare_exchange_strong(a.level + v, &expect, a.L, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                     __HIP_MEMORY_SCOPE_AGENT);
+      uint32_t end = dg;  // inclusive prefix over the lanes
+#pragma unroll
+      for (int d = 1; d < kWave; d <<= 1) {
+        const uint32_t t = __shfl_up(end, d);
+        if (lane >= (uint32_t)d) end += t;
+      }
+      const uint32_t total = readlane(end, kWave - 1);
+      for (uint32_t b0 = 0; b0 < total; b0 += kWave) {
+        const uint32_t f = b0 + lane;
+        // owner: the number of lanes whose range ends at or before f
+        uint32_t k = 0;
+#pragma unroll
+        for (uint32_t step = 32; step >= 1; step >>= 1)
+          if ((uint32_t)__shfl(end, (int)(k + step - 1)) <= f) k += step;
+        const uint32_t ks = __shfl(rs, (int)k), kend = __shfl(end, (int)k), kdg = __shfl(dg, (int)k);
+        const bool valid = f < total;
+        pdo_visit(a, L, valid, valid ? a.edge_dst[ks + (f - (kend - kdg))] : 0u, cnt, deg);
       }
     }
-    do_push(a, found, v);
   }
 }
 
-// Bottom-up level L: every unvisited vertex (one per lane) looks for a parent
-// at level L-1 among its in-edges and stops at the first.  A lane reads 4
-// in-edges, then tests their sources' bits in the previous level's bitmap,
-// so each step has 4 loads in flight.  Levels written earlier in this launch
-// are L, never L-1, so racing reads cannot mislead the search.  (Measured on
-// RMAT-20: 8 lanes per vertex with coalesced in-edge reads was 4x slower --
-// each wavefront then searches 8 vertices instead of 64.)
-//
-// A wavefront owns 64 consecutive vertices (64-aligned), so it writes their
-// two words of this level's bitmap with plain stores -- every word of the
-// bitmap is written, and no atomics are needed.  The level's vertex count and
-// out-edge count are summed per block into partial[2 b], partial[2 b + 1]
-// (bu_reduce_kernel adds them up): one counter that every wavefront with a
-// discovery bumps serialises the whole level (RMAT-20: 2 x 16K same-address
-// atomics took ~240 us per level).  Nothing is queued: a top-down level that
-// follows compacts the bitmap into its queue (bitmap_queue_kernel).
-constexpr uint32_t kBuRun = 4;
-__global__ __launch_bounds__(kDoBlock) void bottom_up_kernel(DoArgs a, uint32_t *partial) {
-  __shared__ uint32_t s_cnt, s_deg;
-  if (threadIdx.x == 0) s_cnt = s_deg = 0;
-  clear_next(a);
-  __syncthreads();
-  const uint32_t *prev = a.fb[(a.L - 1) % 3];
-  uint32_t *cur = a.fb[a.L % 3];
-  const uint32_t words = (a.nnodes + 31) / 32;
-  const uint32_t lane = lane_id();
+// Bottom-up level L: one unvisited vertex per lane searches its in-edges for
+// a parent in the previous level's bitmap; a wavefront owns 64 consecutive
+// vertices and stores their two words of this level's bitmap.
+__device__ __forceinline__ void pdo_bottom_up(const PdoArgs &a, const Sched &s, uint32_t &cnt, uint32_t &deg) {
+  const uint32_t L = s.round, lane = lane_id();
+  const uint32_t *prev = a.fb[(L - 1) % 3];
+  uint32_t *cur = a.fb[L % 3];
   const size_t nt = (size_t)gridDim.x * blockDim.x;
-  uint32_t cnt = 0, deg = 0;
-  // whole wavefronts iterate together: lane 0's vertex is the first of the 64
   const size_t vend = (a.nnodes + (size_t)kWave - 1) / kWave * kWave;
   for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < vend; v += nt) {
     bool found = false;
-    if (v < a.nnodes && a.level[v] == kInf) {
+    if (v < a.nnodes && ld_coh(a.level + v) == kInf) {
       const uint32_t e1 = a.in_row[v + 1];
       for (uint32_t e = a.in_row[v]; e < e1 && !found; e += kBuRun) {
         uint32_t u[kBuRun];
@@ -271,47 +281,32 @@ __global__ __launch_bounds__(kDoBlock) void bottom_up_kernel(DoArgs a, uint32_t 
         for (uint32_t k = 0; k < kBuRun; ++k) u[k] = e + k < e1 ? a.in_src[e + k] : a.nnodes;
 #pragma unroll
         for (uint32_t k = 0; k < kBuRun; ++k)
-          found = found || (u[k] < a.nnodes && ((prev[u[k] >> 5] >> (u[k] & 31)) & 1u));
+          found = found || (u[k] < a.nnodes && ((ld_coh(prev + (u[k] >> 5)) >> (u[k] & 31)) & 1u));
       }
-      if (found) {
-        a.level[v] = a.L;
-        cnt += 1;
-        deg += a.row_start[v + 1] - a.row_start[v];
-      }
+      if (found) st_coh(a.level + v, L);
     }
     const uint64_t m = __ballot(found);
     if (lane == 0) {
-      const uint32_t w0 = (uint32_t)(v / 32);  // v = the wave's first vertex, < nnodes
-      cur[w0] = (uint32_t)m;
-      if (w0 + 1 < words) cur[w0 + 1] = (uint32_t)(m >> 32);
+      const uint32_t w0 = (uint32_t)(v / 32);  // the wave's first vertex, < nnodes
+      st_coh(cur + w0, (uint32_t)m);
+      if (w0 + 1 < a.words) st_coh(cur + w0 + 1, (uint32_t)(m >> 32));
     }
-  }
-#pragma unroll
-  for (int d = kWave / 2; d >= 1; d >>= 1) {
-    cnt += __shfl_xor(cnt, d);
-    deg += __shfl_xor(deg, d);
-  }
-  if (lane == 0 && cnt) {
-    atomicAdd(&s_cnt, cnt);
-    atomicAdd(&s_deg, deg);
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    partial[2 * blockIdx.x] = s_cnt;
-    partial[2 * blockIdx.x + 1] = s_deg;
+    pdo_found(a, L, found, (uint32_t)v, cnt, deg);
   }
 }
 
-// One block: out[0] = sum of partial[2 b], out[1] = sum of partial[2 b + 1]
-// over n blocks.
-__global__ __launch_bounds__(kDoBlock) void bu_reduce_kernel(const uint32_t *partial, uint32_t n, uint32_t *out) {
-  __shared__ uint32_t s[2];
-  if (threadIdx.x < 2) s[threadIdx.x] = 0;
+// Barrier leader after level L: totals of the level, the unexplored-edge
+// estimate, and the direction of level L + 1 (the rule of the host loop it
+// replaces: bottom-up once the frontier's edges exceed 1/alpha of the
+// unexplored edges, top-down again below 1/beta of the vertices).
+__device__ void pdo_leader(const PdoArgs &a) {
+  __shared__ uint32_t tot[2];
+  if (threadIdx.x < 2) tot[threadIdx.x] = 0;
   __syncthreads();
   uint32_t c = 0, d = 0;
-  for (uint32_t b = threadIdx.x; b < n; b += blockDim.x) {
-    c += partial[2 * b];
-    d += partial[2 * b + 1];
+  for (uint32_t b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
+    c += ld_coh(a.partial + 2 * b);
+    d += ld_coh(a.partial + 2 * b + 1);
   }
 #pragma unroll
   for (int k = kWave / 2; k >= 1; k >>= 1) {
@@ -319,51 +314,105 @@ __global__ __launch_bounds__(kDoBlock) void bu_reduce_kernel(const uint32_t *par
     d += __shfl_xor(d, k);
   }
   if (lane_id() == 0) {
-    atomicAdd(&s[0], c);
-    atomicAdd(&s[1], d);
+    atomicAdd(&tot[0], c);
+    atomicAdd(&tot[1], d);
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    out[0] = s[0];
-    out[1] = s[1];
-  }
-}
-
-// Queue of the vertices whose bit is set in bitmap fb (words 0 .. words-1):
-// lane i takes word i of a 64-word run, and one atomic per wavefront reserves
-// the run's places in q.
-__global__ void bitmap_queue_kernel(const uint32_t *fb, uint32_t words, uint32_t *q, uint32_t *tail) {
-  const size_t nt = (size_t)gridDim.x * blockDim.x;
-  const size_t iend = ((size_t)words + kWave - 1) / kWave * kWave;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < iend; i += nt) {
-    uint32_t bits = i < words ? fb[i] : 0u;
-    const uint32_t n = (uint32_t)__builtin_popcount(bits);
-    uint32_t pos = wave_reserve(n, tail);
-    while (bits) {
-      const uint32_t b = (uint32_t)__builtin_ctz(bits);
-      q[pos++] = (uint32_t)i * 32 + b;
-      bits &= bits - 1;
+    Ctrl *ctl = a.ctrl;
+    Sched s;
+    sched_load(s, &ctl->s);
+    const uint32_t L = s.round, n_new = tot[0], mf_new = tot[1];
+    s.phase += s.aux[0];  // bottom-up levels so far
+    const uint32_t m_u = s.aux[3] > s.aux[2] ? s.aux[3] - s.aux[2] : 0u;
+    bool bu = s.aux[0] != 0;
+    if (a.direction == 0) {
+      if (!bu && (uint64_t)mf_new * kAlpha > m_u) bu = true;
+      else if (bu && (uint64_t)n_new * kBeta < a.nnodes) bu = false;
     }
+    s.aux[0] = bu ? 1u : 0u;
+    s.aux[1] = n_new;
+    s.aux[2] = mf_new;
+    s.aux[3] = m_u;
+    s.heavy_count = min(ld_coh(&ctl->hcount[L & 1]), a.hcap);
+    st_coh(&ctl->hcount[(L + 1) & 1], 0u);
+    s.round = L + 1;
+    s.done = n_new == 0;
+    sched_store(&ctl->s, s);
   }
 }
 
-__global__ void zero_words_kernel(uint32_t *p, uint32_t n) {
-  if (threadIdx.x < n) p[threadIdx.x] = 0;
+template <int BS>
+__global__ __launch_bounds__(BS) void pdo_kernel(PdoArgs a) {
+  __shared__ Sched ss;
+  __shared__ uint32_t quit;
+  __shared__ PdoShared sh;
+  Ctrl *c = a.ctrl;
+  const uint32_t nb = gridDim.x;
+  if (!residency_handshake(c, nb, a.residency_limit)) return;
+  const uint32_t gwave = (blockIdx.x * BS + threadIdx.x) / kWave, nwaves = nb * (BS / kWave);
+  for (;;) {
+    if (threadIdx.x == 0) {
+      sched_load(ss, &c->s);
+      quit = ss.done || ld_coh(&c->abort) != kAbortNone;
+      sh.cnt = sh.deg = 0;
+    }
+    __syncthreads();
+    if (quit) break;
+    const Sched s = ss;
+    // the bitmap of level L + 1 was last read at level L - 1
+    uint32_t *nxt = a.fb[(s.round + 1) % 3];
+    for (size_t i = blockIdx.x * (size_t)BS + threadIdx.x; i < a.words; i += (size_t)nb * BS) st_coh(nxt + i, 0u);
+    uint32_t cnt = 0, deg = 0;
+    if (s.aux[0]) pdo_bottom_up(a, s, cnt, deg);
+    else pdo_top_down(a, s, gwave, nwaves, cnt, deg);
+#pragma unroll
+    for (int k = kWave / 2; k >= 1; k >>= 1) {
+      cnt += __shfl_xor(cnt, k);
+      deg += __shfl_xor(deg, k);
+    }
+    if (lane_id() == 0 && cnt) {
+      atomicAdd(&sh.cnt, cnt);
+      atomicAdd(&sh.deg, deg);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      st_coh(a.partial + 2 * blockIdx.x, sh.cnt);
+      st_coh(a.partial + 2 * blockIdx.x + 1, sh.deg);
+    }
+    grid_barrier_block(c, nb, a.spin_limit, [&] { pdo_leader(a); });
+  }
 }
 
-// level = INF except src = 0; bitmaps 0 and 1 cleared (2 is cleared by
-// level 1's kernel) and src's bit set in bitmap 0; queue and counters seeded.
-__global__ void do_init_kernel(uint32_t *level, uint32_t V, uint32_t src, uint32_t *q, uint32_t *ctr, uint32_t *fb0,
-                               uint32_t *fb1) {
+// level = INF except src = 0; bitmap 0 = {src}, bitmaps 1 and 2 empty; the
+// control block and the schedule of level 1; src's chunks if it is heavy.
+__global__ void pdo_init_kernel(PdoArgs a, uint32_t src, uint32_t src_deg, uint32_t bottom_up, uint64_t nedges) {
   const size_t nt = (size_t)gridDim.x * blockDim.x, t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  for (size_t v = t; v < V; v += nt) level[v] = v == src ? 0u : kInf;
-  for (size_t i = t; i < (V + 31) / 32; i += nt) {
-    fb0[i] = i == src / 32 ? 1u << (src & 31) : 0u;
-    fb1[i] = 0;
+  for (size_t v = t; v < a.nnodes; v += nt) a.level[v] = v == src ? 0u : kInf;
+  for (size_t i = t; i < a.words; i += nt) {
+    a.fb[0][i] = i == src / 32 ? 1u << (src & 31) : 0u;
+    a.fb[1][i] = 0;
+    a.fb[2][i] = 0;
   }
+  const uint32_t nch = src_deg >= kTdHeavy ? (src_deg + kTdChunk - 1) / kTdChunk : 0u;
+  for (size_t k = t; k < nch; k += nt) a.hq[0][k] = ((uint64_t)k << 32) | src;
+  Ctrl *c = a.ctrl;
+  ctrl_reset_barrier(c, t, nt);
   if (t == 0) {
-    q[0] = src;
-    ctr[0] = ctr[1] = ctr[2] = ctr[3] = 0;
+    c->abort = kAbortNone;
+    c->checkin = 0;
+    c->resident_seen = 0;
+    c->progress = 0;
+    c->hcount[0] = c->hcount[1] = 0;
+    Sched s;
+    memset(&s, 0, sizeof s);
+    s.round = 1;
+    s.heavy_count = nch;
+    s.aux[0] = bottom_up;
+    s.aux[1] = 1;
+    s.aux[2] = src_deg;
+    s.aux[3] = (uint32_t)nedges;
+    c->s = s;
   }
 }
 
@@ -441,90 +490,73 @@ int lsg_bfs_do(const lsg_graph *g, const lsg_graph *gt, uint32_t src, int direct
   std::lock_guard<std::mutex> lk(w->mu);
   HIPCHK(hipSetDevice(w->device));
   const uint32_t V = g->nnodes;
-  void *qbuf, *cbuf, *pbuf;
-  const uint32_t ptiles = (V + kScanTile - 1) / kScanTile;
-  const size_t words = ((size_t)V + 31) / 32;
-  const uint32_t vgrid = (uint32_t)std::min<uint64_t>(((uint64_t)V + 255) / 256, 16ull * w->cus);
-  if ((rc = ws_buffer(w, kBufQueues, 2ull * V * 4 + 3 * words * 4, &qbuf)) || (rc = ws_buffer(w, kBufSeeds, 2048, &cbuf)) ||
-      (rc = ws_buffer(w, kBufSeedList, ((size_t)V + ptiles + 64 + 2ull * vgrid) * 4, &pbuf)))
+  const uint32_t words = (V + 31) / 32;
+  // every vertex is found once, so all chunks over the whole traversal fit:
+  // sum of ceil(deg / kTdChunk) over heavy vertices <= 2 E / kTdChunk
+  const uint64_t hcap64 = 2ull * g->nedges / kTdChunk + 64;
+  if (hcap64 > UINT32_MAX) return fail(LSG_ERR_ARG, "bfs_do: graph too large");
+  const uint32_t hcap = (uint32_t)hcap64;
+  constexpr int kBs = 256;
+  const void *fn = (const void *)pdo_kernel<kBs>;
+  int grid = 0;
+  if ((rc = resident_grid(w, fn, kBs, &grid))) return rc;
+  void *bbuf, *hbuf, *pbuf;
+  if ((rc = ws_buffer(w, kBufQueues, 3ull * words * 4, &bbuf)) || (rc = ws_buffer(w, kBufHeavy, 2ull * hcap * 8, &hbuf)) ||
+      (rc = ws_buffer(w, kBufSeedList, 2ull * kMaxGridBlocks * 4, &pbuf)))
     return rc;
-  uint32_t *q[2] = {(uint32_t *)qbuf, (uint32_t *)qbuf + V};
-  uint32_t *fbits = (uint32_t *)qbuf + 2ull * V;
-  uint32_t *ctr = (uint32_t *)cbuf;  // [0..3] level counts by parity, [4] queue tail of a compaction
-  uint32_t *pre = (uint32_t *)pbuf, *ptile = pre + V + 1, *ptotal = ptile + ptiles;
-  uint32_t *partial = pre + V + ptiles + 64;  // per-block counts of a bottom-up level
   uint32_t rs2[2];
   HIPCHK(hipMemcpy(rs2, g->row_start + src, 8, hipMemcpyDeviceToHost));
-  HIPCHK(hipEventRecord(w->ev0, w->stream));
-  hipLaunchKernelGGL(do_init_kernel, dim3(vgrid), dim3(256), 0, w->stream, (uint32_t *)level_dev, V, src, q[0], ctr,
-                     fbits, fbits + words);
-  DoArgs a;
+  const uint32_t src_deg = rs2[1] - rs2[0];
+  PdoArgs a;
+  memset(&a, 0, sizeof a);
   a.row_start = g->row_start;
   a.edge_dst = g->edge_dst;
   a.in_row = gt->row_start;
   a.in_src = gt->edge_dst;
   a.level = (uint32_t *)level_dev;
-  a.ctr = ctr;
-  for (int i = 0; i < 3; ++i) a.fb[i] = fbits + i * words;
+  for (int i = 0; i < 3; ++i) a.fb[i] = (uint32_t *)bbuf + (size_t)i * words;
+  a.hq[0] = (uint64_t *)hbuf;
+  a.hq[1] = (uint64_t *)hbuf + hcap;
+  a.partial = (uint32_t *)pbuf;
+  a.ctrl = w->ctrl;
   a.nnodes = V;
-  uint64_t m_u = g->nedges, m_f = rs2[1] - rs2[0];
-  uint32_t n_in = 1, L = 1, bu_levels = 0;
-  bool bottom_up = direction == 2, prev_bottom_up = false;
-  while (n_in > 0) {
-    if (direction == 0) {
-      // Beamer's rule: go bottom-up when the frontier's edges exceed the
-      // unexplored edges / alpha; return once the frontier is small again
-      if (!bottom_up && m_f * kAlpha > m_u) bottom_up = true;
-      else if (bottom_up && (uint64_t)n_in * kBeta < V) bottom_up = false;
-    }
-    a.q_in = q[(L - 1) & 1];
-    a.q_out = q[L & 1];
-    a.n_in = n_in;
-    a.L = L;
-    if (bottom_up) {
-      hipLaunchKernelGGL(bottom_up_kernel, dim3(vgrid), dim3(kDoBlock), 0, w->stream, a, partial);
-      hipLaunchKernelGGL(bu_reduce_kernel, dim3(1), dim3(kDoBlock), 0, w->stream, (const uint32_t *)partial, vgrid,
-                         ctr + 2 * (L & 1));
-      ++bu_levels;
-    } else {
-      if (prev_bottom_up) {
-        // a bottom-up level leaves its vertices in its bitmap only
-        const uint32_t wgrid = (uint32_t)std::min<uint64_t>((words + 255) / 256, 16ull * w->cus);
-        hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, w->stream, ctr + 4, 1u);
-        hipLaunchKernelGGL(bitmap_queue_kernel, dim3(wgrid), dim3(256), 0, w->stream,
-                           (const uint32_t *)a.fb[(L - 1) % 3], (uint32_t)words, q[(L - 1) & 1], ctr + 4);
-      }
-      const uint32_t nt = (n_in + kScanTile - 1) / kScanTile;
-      const uint32_t g1 = (uint32_t)std::min<uint64_t>(((uint64_t)n_in + 255) / 256, 16ull * w->cus);
-      hipLaunchKernelGGL(frontier_deg_kernel, dim3(g1), dim3(256), 0, w->stream, a.q_in, n_in, a.row_start, pre);
-      hipLaunchKernelGGL(scan_tile_kernel, dim3(nt), dim3(kDoBlock), 0, w->stream, pre, n_in, pre, ptile);
-      hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(kDoBlock), 0, w->stream, ptile, nt, ptotal);
-      hipLaunchKernelGGL(scan_add_kernel, dim3(g1), dim3(256), 0, w->stream, pre, n_in, ptile, (uint32_t *)nullptr,
-                         ptotal);
-      const uint32_t g2 = (uint32_t)std::min<uint64_t>((m_f + kDoBlock - 1) / kDoBlock + 1, 16ull * w->cus);
-      hipLaunchKernelGGL(top_down_kernel, dim3(g2), dim3(kDoBlock), 0, w->stream, a, (const uint32_t *)pre);
-    }
-    HIPCHK(hipGetLastError());
-    uint32_t *c = w->pinned;  // pinned: the per-level readback is one short DMA
-    HIPCHK(hipMemcpyAsync(c, ctr + 2 * (L & 1), 8, hipMemcpyDeviceToHost, w->stream));
-    HIPCHK(hipStreamSynchronize(w->stream));
-    prev_bottom_up = bottom_up;
-    m_u = m_u > m_f ? m_u - m_f : 0;
-    n_in = c[0];
-    m_f = c[1];
-    ++L;
-  }
-  HIPCHK(hipEventRecord(w->ev1, w->stream));
-  HIPCHK(hipEventSynchronize(w->ev1));
+  a.words = words;
+  a.hcap = hcap;
+  a.direction = (uint32_t)direction;
+  a.spin_limit = spin_limit_ticks(w);
+  a.residency_limit = residency_limit_ticks(w);
+  // direction of level 1: the frontier is the source alone
+  const uint32_t bu1 = direction == 2 || (direction == 0 && (uint64_t)src_deg * kAlpha > g->nedges);
+  const uint32_t igrid = (uint32_t)std::min<uint64_t>(((uint64_t)V + 255) / 256, 8ull * w->cus);
+  Ctrl hc;
   float ms = 0;
-  HIPCHK(hipEventElapsedTime(&ms, w->ev0, w->ev1));
+  for (int attempt = 0;; ++attempt) {
+    HIPCHK(hipEventRecord(w->ev0, w->stream));
+    hipLaunchKernelGGL(pdo_init_kernel, dim3(igrid), dim3(256), 0, w->stream, a, src, src_deg, bu1,
+                       (uint64_t)g->nedges);
+    hipLaunchKernelGGL(pdo_kernel<kBs>, dim3(grid), dim3(kBs), 0, w->stream, a);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(w->ev1, w->stream));
+    HIPCHK(hipMemcpyAsync(&hc, w->ctrl, kCtrlHostBytes, hipMemcpyDeviceToHost, w->stream));
+    HIPCHK(hipStreamSynchronize(w->stream));
+    HIPCHK(hipEventElapsedTime(&ms, w->ev0, w->ev1));
+    if (hc.abort != kAbortResidency) break;
+    if (attempt > 0 || hc.resident_seen == 0 || hc.resident_seen >= (uint32_t)grid)
+      return fail(LSG_ERR_HIP, "bfs_do: persistent grid of %d x %d blocks is not co-resident (%u were)", grid, kBs,
+                  hc.resident_seen);
+    grid = (int)hc.resident_seen;
+    record_resident_grid(w, fn, kBs, grid);
+  }
+  if (hc.abort == kAbortTimeout)
+    return fail(LSG_ERR_TIMEOUT, "bfs_do: grid barrier timed out at level %u; levels are incomplete", hc.s.round);
+  if (hc.abort == kAbortCapacity) return fail(LSG_ERR_CAPACITY, "bfs_do: chunk queue capacity exceeded");
   if (stats) {
     memset(stats, 0, sizeof *stats);
     stats->ms_kernel = ms;
-    stats->iterations = L - 2;  // levels with a vertex (the last launch finds nothing)
-    stats->near_iters = bu_levels;
-    stats->grid_blocks = (int32_t)vgrid;
-    stats->block_size = kDoBlock;
+    stats->iterations = hc.s.round - 2;  // levels with a vertex (the last level finds nothing)
+    stats->near_iters = hc.s.phase;      // bottom-up levels
+    stats->grid_blocks = grid;
+    stats->block_size = kBs;
     stats->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }
   return LSG_OK;
