@@ -88,7 +88,7 @@ struct MArgs {
 // on a tail that every wavefront of the grid shares serialises at the
 // memory-side atomic unit (~12 ns each on gfx950).
 #ifndef LSG_PUSH_BUF
-#define LSG_PUSH_BUF 256
+#define LSG_PUSH_BUF 512
 #endif
 constexpr uint32_t kPushBuf = LSG_PUSH_BUF;
 
@@ -228,9 +228,6 @@ __device__ __forceinline__ void relax_batch(const MArgs &a, const Sched &s, Lane
 // 16 wide 34.2 (pipelined or not; pipelined it spills).  Improvements are
 // collected in a per-wavefront list and the destinations' masks are updated
 // 64 at a time.
-#ifndef LSG_MULTI_STATIC
-#define LSG_MULTI_STATIC 0
-#endif
 #ifndef LSG_TICKET_SPAN
 #define LSG_TICKET_SPAN 2
 #endif
@@ -686,10 +683,6 @@ void multi_kernel(MArgs a, MultiCounters *tot) {
     // Static first pass (ticket = wavefront id), then dynamic tickets; the
     // next ticket is fetched while the current one is processed.
     const uint32_t nchunks = (s.in_count + kItems - 1) / kItems;
-#if LSG_MULTI_STATIC
-    for (uint32_t ck = wave; ck < nchunks; ck += nwaves) process_items(a, s, ls, ck * kItems, dl, mc, p);
-    for (uint32_t ck = wave; ck < s.heavy_count; ck += nwaves) process_chunk(a, s, ls, ck, mc, p);
-#else
     // A ticket covers kSpan consecutive work units; the next ticket is taken
     // when the last unit of the current range starts.
     for (uint32_t ck = wave, cend = wave + 1; ck < nchunks;) {
@@ -716,7 +709,6 @@ void multi_kernel(MArgs a, MultiCounters *tot) {
         ++ck;
       }
     }
-#endif
     flush_pending(a, s, ls, p);
 #if LSG_PUSH_BUF
     push_flush(a, s, ls, 0);

@@ -97,13 +97,18 @@ def c1(repeats: int) -> dict:
     ref = oracle.dijkstra(g.row_start, g.edge_dst, g.edge_data, 0)
     dev = g.copy_to_gpu()
     buf = DeviceArray(g.nnodes * 4)
-    o = Options(delta=32, start_node=0)
-    sssp_into(dev, buf.ptr, o)
-    assert np.array_equal(buf.to_numpy(np.int32, g.nnodes), ref), "C1 distances differ from Dijkstra"
     er, vr = reach(g, ref)
-    ms = timed_plain(dev, buf, o, repeats)
-    out = {"config": "C1", "graph": "grid256 w[1,100]", "V": g.nnodes, "E": g.nedges, "E_r": er,
-           "plain": rate(er, vr, ms), "cpu": cpu_nearfar(g, [0], 32)}
+    out = {"config": "C1", "graph": "grid256 w[1,100]", "V": g.nnodes, "E": g.nedges, "E_r": er}
+    variants = [("plain", {})]
+    if have_model("plain"):
+        variants.append(("learned", {"sched": 1}))
+    for name, kw in variants:
+        o = Options(delta=32, start_node=0, **kw)
+        st = sssp_into(dev, buf.ptr, o)
+        assert np.array_equal(buf.to_numpy(np.int32, g.nnodes), ref), f"C1 {name} distances differ from Dijkstra"
+        out[name] = dict(rate(er, vr, timed_plain(dev, buf, o, repeats)), phases=st["iterations"],
+                         rounds=st["near_iters"])
+    out["cpu"] = cpu_nearfar(g, [0], 32)
     buf.close()
     dev.close()
     return out
@@ -215,8 +220,8 @@ def c5(repeats: int) -> dict:
     D = DeviceArray(V * 32 * 4)
     buf = DeviceArray(V * 4)
     learned = have_model("batched")
-    bo = Options(delta=32, sched=1, heavy_deg=512, block_size=512) if learned else \
-        Options(delta=1 << 30, far_mode=2, heavy_deg=512, block_size=512)
+    bo = Options(delta=32, sched=1, heavy_deg=256, block_size=256) if learned else \
+        Options(delta=1 << 30, far_mode=0, heavy_deg=256, block_size=256)
     sssp_multi_into(dev, srcs, bo, D)
     Dh = D.to_numpy(np.int32, (V, 32))
     er = vr = 0
