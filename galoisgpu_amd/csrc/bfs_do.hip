@@ -138,7 +138,7 @@ __global__ void scatter_in_kernel(const uint32_t *row_start, const uint32_t *edg
       const uint32_t v = edge_dst[e];
       const uint32_t pos = atomic_add(cursor + v, 1u);
       in_src[pos] = u;
-      if (edge_data) in_w[pos] = edge_data[e];
+      in_w[pos] = edge_data ? edge_data[e] : 0u;
     }
   }
 }

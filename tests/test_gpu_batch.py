@@ -127,6 +127,28 @@ def test_multi_partial_width_and_batch(rmat16):
     dev.close()
 
 
+def test_multi_sources_without_out_edges(rmat16):
+    """Sources without out-edges take no lane (the others run in a compact matrix that is
+    then expanded): every column must still be the oracle's, for a mix of such sources and
+    others, for only such sources, and with one repeated."""
+    g, srcs, _ = rmat16
+    deg = np.diff(g.row_start.astype(np.int64))
+    dead = np.flatnonzero(deg == 0)[:20]
+    live = np.flatnonzero(deg > 0)[:30]
+    assert len(dead) == 20 and len(live) == 30
+    mixed = np.empty(50, dtype=np.int64)
+    mixed[0::5] = dead[:10]
+    mixed[np.arange(50) % 5 != 0] = np.concatenate([live, dead[10:20]])
+    mixed[7] = mixed[0]  # a repeated source without out-edges
+    dev = g.copy_to_gpu()
+    for name, ss in (("mixed", mixed), ("only without out-edges", dead[:9]), ("one", dead[:1]),
+                     ("64 mixed", np.concatenate([mixed, live[:14]]))):
+        say("multi dead sources", name, len(ss))
+        d, st = sssp_multi(dev, ss, Options(delta=32))
+        assert np.array_equal(d, oracle_cols(g, ss)), name
+    dev.close()
+
+
 def test_partial_width_rows_up_to_the_last_vertex():
     """Fewer than 64 sources on a graph whose last vertex is reached: rows of
     D are S words, and relaxing edges into the last row exercises the row

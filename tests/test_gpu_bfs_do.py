@@ -127,3 +127,26 @@ def test_bfs_do_edge_cases():
     lv.close()
     rev.close()
     dev.close()
+
+
+def test_bfs_do_barrier_timeout_is_reported(monkeypatch):
+    """A zero spin limit, set from the host, makes the persistent launch's grid barrier give
+    up at once: every wave still exits and the call reports LSG_ERR_TIMEOUT instead of hanging;
+    the next call with the default limit is exact."""
+    g = CSRGraph.rmat(12, 8, seed=5)
+    src = g.max_degree_vertex()
+    ref, _ = oracle.bfs(g.row_start, g.edge_dst, src)
+    dev = g.copy_to_gpu()
+    rev = dev.transpose()
+    lv = DeviceArray(g.nnodes * 4)
+    monkeypatch.setenv("LSG_BARRIER_TIMEOUT_S", "0")
+    say("bfs_do forced barrier timeout")
+    with pytest.raises(LsgError) as e:
+        bfs_do_into(dev, rev, lv.ptr, src, BFS_AUTO)
+    assert e.value.name == "LSG_ERR_TIMEOUT"
+    monkeypatch.delenv("LSG_BARRIER_TIMEOUT_S")
+    bfs_do_into(dev, rev, lv.ptr, src, BFS_AUTO)
+    assert np.array_equal(lv.to_numpy(np.int32, g.nnodes), ref)
+    lv.close()
+    rev.close()
+    dev.close()
