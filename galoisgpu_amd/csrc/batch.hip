@@ -1095,14 +1095,14 @@ int lsg_sssp_batch(const lsg_graph *g, const uint32_t *srcs, uint32_t nsrc, cons
   HIPCHK(hipSetDevice(w->device));
   uint32_t S = std::min<uint32_t>(nsrc, 64);
   void *dbuf, *pbuf;
-  if ((rc = ws_buffer(w, kBufTranspose, (size_t)g->nnodes * S * 4 + 64 * sizeof(void *), &dbuf))) return rc;
+  if ((rc = ws_buffer(w, kBufUnion, (size_t)g->nnodes * S * 4 + 64 * sizeof(void *), &dbuf))) return rc;
   pbuf = (char *)dbuf + (size_t)g->nnodes * S * 4;
   if (stats) memset(stats, 0, sizeof *stats);
   int launches = 0;
   for (uint32_t base = 0; base < nsrc; base += 64) {
     uint32_t n = std::min<uint32_t>(64, nsrc - base);
     MultiRun r;
-    if ((rc = run_multi(w, g, srcs + base, n, o, (uint32_t *)dbuf, &r))) return rc;
+    if ((rc = run_multi_live(w, g, srcs + base, n, o, (uint32_t *)dbuf, &r))) return rc;
     ++launches;
     HIPCHK(hipMemcpyAsync(pbuf, dist_out + base, n * sizeof(void *), hipMemcpyHostToDevice, w->stream));
     uint32_t blocks = (uint32_t)std::min<uint64_t>(((uint64_t)g->nnodes * n + 255) / 256, 16ull * w->cus);
