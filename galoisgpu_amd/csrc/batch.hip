@@ -810,7 +810,8 @@ __global__ void source_degree_kernel(const uint32_t *row_start, const uint32_t *
 
 // D [V][S] from the compact Dc [V][Sc] of the sources that have out-edges:
 // column s is Dc's column col[s], or, for a source without out-edges, INF
-// except 0 at the source itself.  One wavefront per row, lane = column.
+// except 0 at the source itself.  One wavefront per row, lane = column (8
+// rows per wavefront step measured slower: 0.63 vs 0.48 ms on RMAT-22).
 __global__ void multi_expand(const uint32_t *Dc, uint32_t Sc, const int32_t *col, const uint32_t *srcs, uint32_t S,
                              uint32_t V, uint32_t *D) {
   const uint32_t lane = lane_id();
