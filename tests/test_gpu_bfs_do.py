@@ -2,8 +2,8 @@
 
 Levels must equal the oracle's level-synchronous BFS (apps/bfs/kernel.cu:
 312-334) bit for bit under every direction policy: automatic, top-down only
-and bottom-up only.  The reverse CSR is checked against a numpy transpose row
-by row (as multisets: its rows come out of atomics, unordered).
+and bottom-up only.  The reverse CSR is checked against a numpy stable sort of the
+edges by destination, array for array.
 """
 import ctypes
 import sys
@@ -65,9 +65,33 @@ def test_transpose_matches_numpy():
     sources = np.repeat(np.arange(V), np.diff(r))
     indeg = np.bincount(col, minlength=V)
     assert np.array_equal(row, np.concatenate([[0], np.cumsum(indeg)]).astype(np.uint32))
-    fwd = sorted(zip(col.tolist(), sources.tolist(), g.edge_data.astype(np.int64).tolist()))
-    back = sorted((v, int(src[e]), int(w[e])) for v in range(V) for e in range(row[v], row[v + 1]))
-    assert fwd == back
+    # each row lists its in-edges in forward order (a stable sort of the edges by destination)
+    order = np.argsort(col, kind="stable")
+    assert np.array_equal(src, sources[order].astype(np.uint32))
+    assert np.array_equal(w, g.edge_data[order].astype(np.uint32))
+    # the same arrays on a second call
+    rev2 = dev.transpose()
+    assert np.array_equal(fetch(ctypes.cast(rev2._g.edge_dst, ctypes.c_void_p).value, E), src)
+    rev2.close()
+    rev.close()
+    dev.close()
+
+
+@pytest.mark.parametrize("scale,ef", [(8, 2), (17, 8)])
+def test_transpose_larger_graphs(scale, ef):
+    # two and three radix passes, many tiles, hub rows of thousands of in-edges
+    g = CSRGraph.rmat(scale, ef, seed=scale)
+    dev = g.copy_to_gpu()
+    rev = dev.transpose()
+    V, E = g.nnodes, g.nedges
+    say("transpose rmat", scale, ef)
+    row = fetch(ctypes.cast(rev._g.row_start, ctypes.c_void_p).value, V + 1)
+    src = fetch(ctypes.cast(rev._g.edge_dst, ctypes.c_void_p).value, E)
+    col = g.edge_dst.astype(np.int64)
+    order = np.argsort(col, kind="stable")
+    sources = np.repeat(np.arange(V), np.diff(g.row_start.astype(np.int64)))
+    assert np.array_equal(row, np.concatenate([[0], np.cumsum(np.bincount(col, minlength=V))]).astype(np.uint32))
+    assert np.array_equal(src, sources[order].astype(np.uint32))
     rev.close()
     dev.close()
 
