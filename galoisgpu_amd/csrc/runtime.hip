@@ -59,9 +59,11 @@ uint64_t spin_limit_ticks(const Workspace *w) {
   return (uint64_t)(sec * w->wall_khz * 1000.0);
 }
 
-// Blocks of one launch begin within microseconds, so after 50 ms the missing
-// blocks are waiting for a slot.
-uint64_t residency_limit_ticks(const Workspace *w) { return (uint64_t)(0.05 * w->wall_khz * 1000.0); }
+// Blocks of one launch begin within tens of microseconds, so after 5 ms the
+// missing blocks are waiting for a slot.  (A process pays this once per
+// persistent kernel whose occupancy figure is too high; 50 ms made a CLI
+// run's first traversal 50 ms long.)
+uint64_t residency_limit_ticks(const Workspace *w) { return (uint64_t)(0.005 * w->wall_khz * 1000.0); }
 
 int resident_grid(Workspace *w, const void *fn, int bs, int *grid) {
   for (auto &e : w->grids)
