@@ -609,6 +609,11 @@ int lsg_bfs_do(const lsg_graph *g, const lsg_graph *gt, uint32_t src, int direct
   const void *fn = (const void *)pdo_kernel<kBs>;
   int grid = 0;
   if ((rc = resident_grid(w, fn, kBs, &grid))) return rc;
+  // 4 blocks per CU: fewer blocks per barrier and below the 6 per CU that are
+  // co-resident (the occupancy figure says 7, so a full grid costs a failed
+  // residency handshake on the first call). RMAT-20 from the hub: 0.142 ms
+  // (1024 blocks) vs 0.156 ms (1536) vs 0.148 ms (512).
+  grid = std::min<int>(grid, 4 * (int)w->cus);
   void *bbuf, *hbuf, *pbuf;
   if ((rc = ws_buffer(w, kBufQueues, 3ull * words * 4, &bbuf)) || (rc = ws_buffer(w, kBufHeavy, 2ull * hcap * 8, &hbuf)) ||
       (rc = ws_buffer(w, kBufSeedList, 2ull * kMaxGridBlocks * 4, &pbuf)))

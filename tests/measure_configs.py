@@ -132,6 +132,12 @@ def c2(repeats: int) -> dict:
         bfs_into(dev, buf.ptr, bo)
         assert np.array_equal(buf.to_numpy(np.int32, g.nnodes), lref), f"C2 BFS {name} differs"
         out[f"bfs_{name}"] = rate(er, vr, timed_plain(dev, buf, bo, repeats, bfs=True), bfs=True)
+    if have_model("plain"):
+        o = Options(delta=32, start_node=src, expand_mode=1, sched=1)
+        st = sssp_into(dev, buf.ptr, o)
+        assert np.array_equal(buf.to_numpy(np.int32, g.nnodes), ref), "C2 SSSP learned differs"
+        out["sssp_learned"] = dict(rate(er, vr, timed_plain(dev, buf, o, repeats)), phases=st["iterations"],
+                                   rounds=st["near_iters"])
     # direction-optimising BFS; the reverse CSR is built once per graph, timed apart
     t = time.perf_counter()
     rev = dev.transpose()
