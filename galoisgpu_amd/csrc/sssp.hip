@@ -736,13 +736,13 @@ static int run_core(Workspace *w, const CsrView &g, const uint32_t *seeds, uint3
   KernelFn fn = A == Algo::SSSP ? pick_kernel<Algo::SSSP>(bs) : pick_kernel<Algo::BFS>(bs);
   int grid = 0;
   if ((rc = resident_grid(w, (const void *)fn, bs, &grid))) return rc;
-  // Default: 2 blocks per CU at most.  Fewer blocks per grid barrier beat
-  // more parallel tiles on every measured graph (256-thread blocks; full
-  // resident grid 1280 -> 512): RMAT-20 SSSP 2.24 -> 2.05 ms, road 2048^2
-  // 10.2 -> 8.3 ms, RMAT-22 6.31 -> 6.23 ms, grid 256^2 12.9 -> 12.8 ms;
-  // RMAT-20 BFS 0.56 -> 0.60 ms is the one loss.
+  // Default: 2 blocks per CU at most for SSSP, 4 for BFS.  Fewer blocks per
+  // grid barrier beat more parallel tiles (256-thread blocks; full resident
+  // grid 1280 -> 512): RMAT-20 SSSP 2.24 -> 2.05 ms, road 2048^2 10.2 -> 8.3
+  // ms, RMAT-22 6.31 -> 6.23 ms, grid 256^2 12.9 -> 12.8 ms; BFS levels are
+  // wider and cheaper, RMAT-20 BFS 0.56 ms at 1280, 0.55 at 1024, 0.60 at 512.
   if (o.grid_blocks > 0) grid = std::min(o.grid_blocks, grid);
-  else grid = std::min(grid, 2 * (int)w->cus);
+  else grid = std::min(grid, (A == Algo::BFS ? 4 : 2) * (int)w->cus);
 
   Ctrl hc;
   float ms = 0;
