@@ -381,9 +381,11 @@ __device__ void process_heavy(const KArgs &a, const Sched &s, uint32_t c, TileSh
 // finish when both are empty.  At a phase boundary the learned scheduler (when
 // enabled) picks the next phase's delta width and expansion mode; it never
 // changes which vertices are relaxed or how, only when.
+// `s` and `abort_seen` are the caller's LDS words: on return they hold the
+// schedule just stored and the abort flag loaded with the step's counters,
+// so rounds that the same block runs next need not load them again.
 template <Algo A>
-__device__ void leader_step(const KArgs &a) {
-  __shared__ Sched s;
+__device__ void leader_step(const KArgs &a, Sched &s, uint32_t &abort_seen) {
   __shared__ SchedFeatures f;
   __shared__ MlpScratch sc;
   __shared__ uint32_t decide, consumed;
@@ -396,6 +398,7 @@ __device__ void leader_step(const KArgs &a) {
     sched_load(s, &c->s);
     CtrlCounts k;
     counts_load(k, c);
+    abort_seen = ld_coh(&c->abort);
     const uint32_t nnear = k.q(s.near_q), nfar = k.q(s.far_q), nheavy = k.h(1u - s.heavy_in);
     if (nnear > a.qcap || nfar > a.qcap || nheavy > a.hcap) {
       // A push found its queue full (and raised kAbortCapacity); never hand
@@ -503,17 +506,14 @@ __device__ __forceinline__ uint32_t tile_items(uint32_t in_count, uint32_t nbloc
 #endif
 constexpr uint32_t kLocalTiles = LSG_LOCAL_TILES;
 
+// `ls` and `labort`: the schedule and abort flag the leader step just left in LDS.
 template <Algo A, int BS>
-__device__ void local_rounds(const KArgs &a, TileShared<BS> &sh) {
-  __shared__ Sched ls;
+__device__ void local_rounds(const KArgs &a, TileShared<BS> &sh, Sched &ls, uint32_t &labort) {
   __shared__ uint32_t go;
   Ctrl *c = a.ctrl;
   for (;;) {
     if (threadIdx.x == 0) {
-      order_release();  // the schedule this thread just stored is complete before it is read back
-      sched_load(ls, &c->s);
-      go = !ls.done && ls.heavy_count == 0 && ls.in_count <= kLocalTiles * BS &&
-           ld_coh(&c->abort) == kAbortNone;
+      go = !ls.done && ls.heavy_count == 0 && ls.in_count <= kLocalTiles * BS && labort == kAbortNone;
       sh.edges = 0;
     }
     __syncthreads();
@@ -529,7 +529,7 @@ __device__ void local_rounds(const KArgs &a, TileShared<BS> &sh) {
     // this block's pushes and counts are complete before the leader step reads them
     order_release();
     __syncthreads();
-    leader_step<A>(a);
+    leader_step<A>(a, ls, labort);
     if (threadIdx.x == 0) st_coh(&c->progress, s.round);
   }
 }
@@ -538,6 +538,8 @@ template <Algo A, int BS>
 __global__ __launch_bounds__(BS) void persistent_kernel(KArgs a) {
   __shared__ TileShared<BS> sh;
   __shared__ Sched ss;
+  __shared__ Sched lsched;  // the leader step's schedule, reused by local rounds
+  __shared__ uint32_t labort;
   __shared__ uint32_t tk;
   __shared__ uint32_t quit;
   Ctrl *c = a.ctrl;
@@ -579,8 +581,8 @@ __global__ __launch_bounds__(BS) void persistent_kernel(KArgs a) {
         __hip_atomic_fetch_add(&c->stat_edges, (uint64_t)sh.edges, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     grid_barrier_block(c, nb, a.spin_limit, [&] {
-      leader_step<A>(a);
-      local_rounds<A, BS>(a, sh);
+      leader_step<A>(a, lsched, labort);
+      local_rounds<A, BS>(a, sh, lsched, labort);
     });
   }
 }
