@@ -145,21 +145,23 @@ __device__ __forceinline__ void stage_drain(const KArgs &a, const Sched &s, Push
 struct SsspRelax {
   template <int BS>
   __device__ __forceinline__ static void edge(const KArgs &a, const Sched &s, uint32_t v, uint32_t nd, bool active,
-                                              PushStage<BS> &ps) {
+                                              PushStage<BS> &ps, bool precheck) {
     // The pre-checks read through the cache: a stale d[v] is only ever larger
     // than the current one (costing at most a redundant atomicMin, whose
     // return value is authoritative), and a stale stamp is only ever older.
+    // Without `precheck` every edge goes straight to the atomicMin: one
+    // round trip fewer, for rounds too small for the extra atomics to matter.
     bool imp = false;
     const bool near = (int64_t)nd <= s.curdelta();
     uint32_t *stamps = near ? a.near_stamp : a.far_stamp;
     const uint32_t stamp = near ? s.round + 1 : s.phase + 1;
     uint32_t seen = stamp;
     if (active) {
-      uint32_t dv = a.dist[v];
+      const uint32_t dv = precheck ? a.dist[v] : kInf;
 #if LSG_STAMP_PRELOAD
       seen = stamps[v];  // loaded beside d[v]: one round trip fewer when the edge improves
 #endif
-      if (nd < dv) imp = nd < atomic_min(a.dist + v, nd);
+      if (!precheck || nd < dv) imp = nd < atomic_min(a.dist + v, nd);
     }
 #if !LSG_STAMP_PRELOAD
     if (imp) seen = stamps[v];
@@ -181,11 +183,11 @@ struct SsspRelax {
 struct BfsVisit {
   template <int BS>
   __device__ __forceinline__ static void edge(const KArgs &a, const Sched &s, uint32_t v, uint32_t nd, bool active,
-                                              PushStage<BS> &ps) {
+                                              PushStage<BS> &ps, bool precheck) {
     bool imp = false;
     if (active) {
-      uint32_t dv = a.dist[v];
-      if (nd < dv) imp = nd < atomic_min(a.dist + v, nd);
+      const uint32_t dv = precheck ? a.dist[v] : kInf;
+      if (!precheck || nd < dv) imp = nd < atomic_min(a.dist + v, nd);
     }
     if (imp) {
       uint32_t stamp = s.round + 1;
@@ -197,14 +199,14 @@ struct BfsVisit {
 
 template <Algo A, int BS>
 __device__ __forceinline__ void relax_edge(const KArgs &a, const Sched &s, uint32_t e, uint32_t du, bool active,
-                                           PushStage<BS> &ps) {
+                                           PushStage<BS> &ps, bool precheck) {
   uint32_t v = 0, nd = 0;
   if (active) {
     v = a.edge_dst[e];
     nd = A == Algo::SSSP ? du + a.edge_data[e] : du + 1u;
   }
-  if (A == Algo::SSSP) SsspRelax::edge<BS>(a, s, v, nd, active, ps);
-  else BfsVisit::edge<BS>(a, s, v, nd, active, ps);
+  if (A == Algo::SSSP) SsspRelax::edge<BS>(a, s, v, nd, active, ps, precheck);
+  else BfsVisit::edge<BS>(a, s, v, nd, active, ps, precheck);
 }
 
 // ---- block-level helpers ---------------------------------------------------
@@ -257,7 +259,8 @@ struct TileShared {
 // queues heavy vertices as chunks for the next round, and expands the rest by
 // the selected mode.
 template <Algo A, int BS>
-__device__ void process_tile(const KArgs &a, const Sched &s, uint32_t t, uint32_t ti, TileShared<BS> &sh) {
+__device__ void process_tile(const KArgs &a, const Sched &s, uint32_t t, uint32_t ti, TileShared<BS> &sh,
+                             bool precheck = true) {
   uint32_t i = t * ti + threadIdx.x;
   bool valid = threadIdx.x < ti && i < s.in_count;
   uint32_t u = valid ? ld_coh(a.q[s.in_q] + i) : 0u;
@@ -330,7 +333,7 @@ __device__ void process_tile(const KArgs &a, const Sched &s, uint32_t t, uint32_
         ee = sh.start[lo] + (e - sh.offs[lo]);
         d0 = sh.du[lo];
       }
-      relax_edge<A, BS>(a, s, ee, d0, act, sh.ps);
+      relax_edge<A, BS>(a, s, ee, d0, act, sh.ps, precheck);
     }
     __syncthreads();
   } else {
@@ -341,7 +344,7 @@ __device__ void process_tile(const KArgs &a, const Sched &s, uint32_t t, uint32_
     uint32_t mx = my;
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, off));
-    for (uint32_t j = 0; j < mx; ++j) relax_edge<A, BS>(a, s, rs + j, du, j < my, sh.ps);
+    for (uint32_t j = 0; j < mx; ++j) relax_edge<A, BS>(a, s, rs + j, du, j < my, sh.ps, precheck);
     uint64_t med = __ballot(!small && deg > 0);
     while (med) {
       uint32_t l = (uint32_t)__builtin_ctzll(med);
@@ -349,7 +352,7 @@ __device__ void process_tile(const KArgs &a, const Sched &s, uint32_t t, uint32_
       uint32_t ws = readlane(rs, l), wd = readlane(deg, l), wu = readlane(du, l);
       for (uint32_t j = 0; j < wd; j += kWave) {
         uint32_t jj = j + lane_id();
-        relax_edge<A, BS>(a, s, ws + jj, wu, jj < wd, sh.ps);
+        relax_edge<A, BS>(a, s, ws + jj, wu, jj < wd, sh.ps, precheck);
       }
     }
   }
@@ -370,7 +373,7 @@ __device__ void process_heavy(const KArgs &a, const Sched &s, uint32_t c, TileSh
   if ((a.sched || a.feat) && threadIdx.x == 0) atomicAdd(&sh.edges, (unsigned long long)(hi - lo));
   for (uint32_t b = lo; b < hi; b += BS) {
     uint32_t e = b + threadIdx.x;
-    relax_edge<A, BS>(a, s, e, du, e < hi, sh.ps);
+    relax_edge<A, BS>(a, s, e, du, e < hi, sh.ps, true);
   }
 }
 
@@ -504,6 +507,9 @@ __device__ __forceinline__ uint32_t tile_items(uint32_t in_count, uint32_t nbloc
 #ifndef LSG_LOCAL_TILES
 #define LSG_LOCAL_TILES 1
 #endif
+#ifndef LSG_LOCAL_PRECHECK
+#define LSG_LOCAL_PRECHECK 0
+#endif
 constexpr uint32_t kLocalTiles = LSG_LOCAL_TILES;
 
 // `ls` and `labort`: the schedule and abort flag the leader step just left in LDS.
@@ -520,7 +526,7 @@ __device__ void local_rounds(const KArgs &a, TileShared<BS> &sh, Sched &ls, uint
     if (!go) return;
     const Sched s = ls;
     for (uint32_t t = 0; t * BS < s.in_count; ++t) {
-      process_tile<A, BS>(a, s, t, BS, sh);
+      process_tile<A, BS>(a, s, t, BS, sh, LSG_LOCAL_PRECHECK != 0);
       __syncthreads();
     }
     stage_drain<BS>(a, s, sh.ps);
