@@ -84,6 +84,10 @@ template <int BS>
 struct PushStage {
   uint32_t item[BS / kWave][2][kStage];  // [wave][near, far]
   uint32_t n[BS / kWave][2];
+  // In a block-local round this block is the only one pushing: the piles'
+  // tails are kept here and stored once at the end of the round.
+  uint32_t local;
+  uint32_t tail[2];
 };
 
 // Writes this wavefront's staged items of pile `which` (0 near, 1 far) to the queue.
@@ -94,7 +98,7 @@ __device__ __forceinline__ void stage_flush(const KArgs &a, const Sched &s, Push
   if (n == 0) return;
   const uint32_t qi = which ? s.far_q : s.near_q;
   uint32_t base = 0;
-  if (lane_id() == 0) base = atomic_add(&a.ctrl->qcount[qi], n);
+  if (lane_id() == 0) base = ps.local ? atomicAdd(&ps.tail[which], n) : atomic_add(&a.ctrl->qcount[qi], n);
   base = readlane(base, 0);
   for (uint32_t i = lane_id(); i < n; i += kWave) {
     const uint32_t pos = base + i;
@@ -384,11 +388,12 @@ __device__ void process_heavy(const KArgs &a, const Sched &s, uint32_t c, TileSh
 // finish when both are empty.  At a phase boundary the learned scheduler (when
 // enabled) picks the next phase's delta width and expansion mode; it never
 // changes which vertices are relaxed or how, only when.
-// `s` and `abort_seen` are the caller's LDS words: on return they hold the
-// schedule just stored and the abort flag loaded with the step's counters,
-// so rounds that the same block runs next need not load them again.
+// `s`, `abort_seen` and `far_tail` are the caller's LDS words: on return they
+// hold the schedule just stored, the abort flag loaded with the step's
+// counters and the far pile's count for the next round (its near pile starts
+// empty), so rounds that the same block runs next need not load them again.
 template <Algo A>
-__device__ void leader_step(const KArgs &a, Sched &s, uint32_t &abort_seen) {
+__device__ void leader_step(const KArgs &a, Sched &s, uint32_t &abort_seen, uint32_t &far_tail) {
   __shared__ SchedFeatures f;
   __shared__ MlpScratch sc;
   __shared__ uint32_t decide, consumed;
@@ -403,6 +408,7 @@ __device__ void leader_step(const KArgs &a, Sched &s, uint32_t &abort_seen) {
     counts_load(k, c);
     abort_seen = ld_coh(&c->abort);
     const uint32_t nnear = k.q(s.near_q), nfar = k.q(s.far_q), nheavy = k.h(1u - s.heavy_in);
+    far_tail = 0;
     if (nnear > a.qcap || nfar > a.qcap || nheavy > a.hcap) {
       // A push found its queue full (and raised kAbortCapacity); never hand
       // out a count beyond the buffer.
@@ -437,6 +443,7 @@ __device__ void leader_step(const KArgs &a, Sched &s, uint32_t &abort_seen) {
         st_coh(&c->qcount[consumed], 0u);
         s.aux[0] = 0;
         s.round += 1;
+        far_tail = nfar;
       } else if (A == Algo::SSSP && nfar > 0) {
         decide = 1;
         s.in_count = nfar;
@@ -514,13 +521,16 @@ constexpr uint32_t kLocalTiles = LSG_LOCAL_TILES;
 
 // `ls` and `labort`: the schedule and abort flag the leader step just left in LDS.
 template <Algo A, int BS>
-__device__ void local_rounds(const KArgs &a, TileShared<BS> &sh, Sched &ls, uint32_t &labort) {
+__device__ void local_rounds(const KArgs &a, TileShared<BS> &sh, Sched &ls, uint32_t &labort, uint32_t &lfar) {
   __shared__ uint32_t go;
   Ctrl *c = a.ctrl;
   for (;;) {
     if (threadIdx.x == 0) {
       go = !ls.done && ls.heavy_count == 0 && ls.in_count <= kLocalTiles * BS && labort == kAbortNone;
       sh.edges = 0;
+      sh.ps.local = go;
+      sh.ps.tail[0] = 0;  // the near pile starts empty
+      sh.ps.tail[1] = lfar;
     }
     __syncthreads();
     if (!go) return;
@@ -530,12 +540,17 @@ __device__ void local_rounds(const KArgs &a, TileShared<BS> &sh, Sched &ls, uint
       __syncthreads();
     }
     stage_drain<BS>(a, s, sh.ps);
-    if (threadIdx.x == 0 && (a.sched || a.feat) && sh.edges)
-      __hip_atomic_fetch_add(&c->stat_edges, (uint64_t)sh.edges, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();  // every wavefront has taken its places from the LDS tails
+    if (threadIdx.x == 0) {
+      st_coh(&c->qcount[s.near_q], sh.ps.tail[0]);
+      st_coh(&c->qcount[s.far_q], sh.ps.tail[1]);
+      if ((a.sched || a.feat) && sh.edges)
+        __hip_atomic_fetch_add(&c->stat_edges, (uint64_t)sh.edges, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
     // this block's pushes and counts are complete before the leader step reads them
     order_release();
     __syncthreads();
-    leader_step<A>(a, ls, labort);
+    leader_step<A>(a, ls, labort, lfar);
     if (threadIdx.x == 0) st_coh(&c->progress, s.round);
   }
 }
@@ -545,13 +560,14 @@ __global__ __launch_bounds__(BS) void persistent_kernel(KArgs a) {
   __shared__ TileShared<BS> sh;
   __shared__ Sched ss;
   __shared__ Sched lsched;  // the leader step's schedule, reused by local rounds
-  __shared__ uint32_t labort;
+  __shared__ uint32_t labort, lfar;
   __shared__ uint32_t tk;
   __shared__ uint32_t quit;
   Ctrl *c = a.ctrl;
   const uint32_t nb = gridDim.x;
   if (!residency_handshake(c, nb, a.residency_limit)) return;
   if (threadIdx.x < 2 * (BS / kWave)) (&sh.ps.n[0][0])[threadIdx.x] = 0;
+  if (threadIdx.x == 0) sh.ps.local = 0;
   for (;;) {
     if (threadIdx.x == 0) {
       Sched s;
@@ -587,8 +603,8 @@ __global__ __launch_bounds__(BS) void persistent_kernel(KArgs a) {
         __hip_atomic_fetch_add(&c->stat_edges, (uint64_t)sh.edges, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     grid_barrier_block(c, nb, a.spin_limit, [&] {
-      leader_step<A>(a, lsched, labort);
-      local_rounds<A, BS>(a, sh, lsched, labort);
+      leader_step<A>(a, lsched, labort, lfar);
+      local_rounds<A, BS>(a, sh, lsched, labort, lfar);
     });
   }
 }
