@@ -70,7 +70,8 @@ class LsgOpts(ctypes.Structure):
         ("sched", c_int32),
         ("trace", c_int32),
         ("graph_launch", c_int32),
-        ("reserved", c_int32 * 4),
+        ("lb_deg", c_int32),
+        ("reserved", c_int32 * 3),
     ]
 
 

@@ -33,6 +33,7 @@ class Options:
     sched: int = 0
     trace: bool = False
     graph_launch: bool = False
+    lb_deg: int = 0             # -l threshold; 0 = the sweep's thread count (DEGREE_LIMIT)
 
     def to_c(self) -> LsgOpts:
         o = LsgOpts()

@@ -27,76 +27,15 @@
 #include "engine.h"
 #include "lsg_internal.h"
 #include "runtime.h"
+#include "scan_dev.h"
 
 namespace lsg {
 
 constexpr int kDoBlock = 256;
-constexpr uint32_t kScanTile = 1024;  // elements per block of the scan kernels
 constexpr uint32_t kScatterRun = 16;  // edges per thread when finding edge sources
 constexpr uint32_t kAlpha = 14, kBeta = 24;
 
 // ---- reverse CSR ------------------------------------------------------------
-
-// Exclusive scan of one tile of kScanTile values per block (4 per thread):
-// out[i] = sum of in[tile start .. i), tile_sum[b] = the tile's total.
-__global__ __launch_bounds__(kDoBlock) void scan_tile_kernel(const uint32_t *in, uint32_t n, uint32_t *out,
-                                                             uint32_t *tile_sum) {
-  __shared__ uint32_t wsum[kDoBlock / kWave];
-  const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * 4;
-  uint32_t x[4], s = 0;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    x[k] = base + k < n ? in[base + k] : 0u;
-    s += x[k];
-  }
-  // inclusive scan of the per-thread sums across the wavefront, then across waves
-  uint32_t inc = s;
-  const uint32_t lane = lane_id(), w = threadIdx.x / kWave;
-#pragma unroll
-  for (int d = 1; d < kWave; d <<= 1) {
-    const uint32_t t = __shfl_up(inc, d);
-    if (lane >= (uint32_t)d) inc += t;
-  }
-  if (lane == kWave - 1) wsum[w] = inc;
-  __syncthreads();
-  uint32_t before = 0;
-  for (uint32_t i = 0; i < w; ++i) before += wsum[i];
-  uint32_t run = before + inc - s;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    if (base + k < n) out[base + k] = run;
-    run += x[k];
-  }
-  if (threadIdx.x == kDoBlock - 1) tile_sum[blockIdx.x] = before + inc;
-}
-
-// One block: exclusive scan of the tile totals in place; *total gets the sum.
-__global__ __launch_bounds__(kDoBlock) void scan_sums_kernel(uint32_t *sums, uint32_t n, uint32_t *total) {
-  __shared__ uint32_t wsum[kDoBlock / kWave];
-  __shared__ uint32_t carry;
-  if (threadIdx.x == 0) carry = 0;
-  __syncthreads();
-  const uint32_t lane = lane_id(), w = threadIdx.x / kWave;
-  for (uint32_t b0 = 0; b0 < n; b0 += kDoBlock) {
-    const uint32_t i = b0 + threadIdx.x;
-    const uint32_t s = i < n ? sums[i] : 0u;
-    uint32_t inc = s;
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-      const uint32_t t = __shfl_up(inc, d);
-      if (lane >= (uint32_t)d) inc += t;
-    }
-    if (lane == kWave - 1) wsum[w] = inc;
-    __syncthreads();
-    uint32_t before = carry;
-    for (uint32_t k = 0; k < w; ++k) before += wsum[k];
-    if (i < n) sums[i] = before + inc - s;
-    __syncthreads();
-    if (threadIdx.x == kDoBlock - 1) carry = before + inc;
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) *total = carry;
-}
 
 // Adds each tile's offset; writes the row starts and the scatter cursors.
 __global__ void scan_add_kernel(uint32_t *row, uint32_t n, const uint32_t *tile_off, uint32_t *cursor,

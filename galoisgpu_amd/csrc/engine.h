@@ -127,6 +127,7 @@ struct Ctrl {
   Sched s;            uint32_t _p6[16];
   uint64_t stat_items, stat_rounds, stat_heavy, stat_phases, stat_edges;
   uint32_t _p7[22];
+  uint32_t far_min;   uint32_t _p8[31];  // bucket far mode: smallest distance sent to the far pile this phase
   uint32_t bar_grp[kMaxBarGroups][32];  // group arrival counters, one line each
 };
 constexpr size_t kCtrlHostBytes = offsetof(Ctrl, bar_grp);
@@ -146,6 +147,7 @@ __device__ __forceinline__ void ctrl_reset_barrier(Ctrl *c, size_t tid, size_t n
 struct CtrlCounts {
   uint32_t qc[3], hc[2];
   uint64_t items, rounds, heavy, phases, edges;
+  uint32_t far_min;
   __device__ uint32_t q(uint32_t i) const { return i == 0 ? qc[0] : i == 1 ? qc[1] : qc[2]; }
   __device__ uint32_t h(uint32_t i) const { return i == 0 ? hc[0] : hc[1]; }
 };
@@ -160,6 +162,7 @@ __device__ __forceinline__ void counts_load(CtrlCounts &k, const Ctrl *c) {
   k.heavy = ld_coh64(&c->stat_heavy);
   k.phases = ld_coh64(&c->stat_phases);
   k.edges = ld_coh64(&c->stat_edges);
+  k.far_min = ld_coh(&c->far_min);
 }
 
 // ---- wave64 helpers --------------------------------------------------------

@@ -29,6 +29,7 @@ enum BufSlot {
   kBufSeeds,
   kBufSeedList,
   kBufUnion,
+  kBufLb,
   kBufCount
 };
 

@@ -1,0 +1,774 @@
+// Device side of the single-source traversal: kernel arguments, the per-edge
+// operators, tile and heavy-chunk expansion, the leader step (pile rotation)
+// and block-local rounds.  Included by sssp.hip (the persistent launch) and
+// lb.hip (the host-driven edge-balanced -l loop), which share every operator
+// so that both compute exactly the same relaxations.
+//
+// Reference path: apps/sssp/kernel.cu (relax operator sssp_kernel_dev
+// :177-306, persistent driver :355-391) and apps/bfs/kernel.cu (operator
+// bfs_kernel_dev :121-274, driver :312-334).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+
+#include <algorithm>
+#include <cstdint>
+
+#include "engine.h"
+#include "lsg_internal.h"
+#include "runtime.h"
+#include "sched_mlp.h"
+
+namespace lsg {
+
+
+enum class Algo : int { SSSP = 0, BFS = 1 };
+
+constexpr uint32_t kTraceCap = 1u << 16;  // round records kept per traced call
+
+struct KArgs {
+  const uint32_t *row_start;
+  const uint32_t *edge_dst;
+  const uint32_t *edge_data;
+  uint32_t *dist;
+  uint32_t *q[3];
+  uint32_t *hq[2];  // heavy chunk queues: (vertex, chunk index) pairs
+  uint32_t *near_stamp;
+  uint32_t *far_stamp;
+  Ctrl *ctrl;
+  uint32_t nnodes;
+  uint32_t qcap;   // capacity of each item queue
+  uint32_t hcap;   // capacity of each heavy queue in chunks
+  uint32_t delta;
+  uint32_t thread_deg;
+  uint32_t heavy_deg;
+  uint32_t expand_mode;
+  uint32_t far_mode;
+  uint64_t spin_limit;       // grid barrier spin bound, wall-clock ticks
+  uint64_t residency_limit;  // start-up handshake bound, wall-clock ticks
+  lsg_round_trace *trace;    // per-round records, or null
+  uint32_t trace_cap;
+  uint32_t sched;            // 1 = learned delta width and expansion mode per phase
+  const float *mlp;          // scheduler weights (plain-kernel model), or null
+  float *feat;               // per-phase feature records [kMaxForced][kFeatRec], or null
+  uint64_t nedges;
+  uint32_t lb_limit;         // -l: degrees >= this are expanded by the edge-balanced pass (else UINT32_MAX)
+};
+
+// ---- pile pushes -----------------------------------------------------------
+
+// Pushes are staged per wavefront in LDS, kStage items per pile, and written
+// out with one queue-tail atomic per flush (and at the end of every round):
+// one atomic per wave_push on a tail that every wavefront of the grid shares
+// serialises at the memory-side atomic unit (~12 ns each on gfx950).
+constexpr uint32_t kStage = 256;
+
+template <int BS>
+struct PushStage {
+  uint32_t item[BS / kWave][2][kStage];  // [wave][near, far]
+  uint32_t n[BS / kWave][2];
+  // Bucket far mode: smallest distance this block sent to the far pile this
+  // round (pushed candidates and deferred items), folded into the control
+  // block's far_min once per round.
+  uint32_t far_min;
+  // In a block-local round this block is the only one pushing: the piles'
+  // tails are kept here and stored once at the end of the round.
+  uint32_t local;
+  uint32_t tail[2];
+};
+
+// Writes this wavefront's staged items of pile `which` (0 near, 1 far) to the queue.
+template <int BS>
+__device__ __forceinline__ void stage_flush(const KArgs &a, const Sched &s, PushStage<BS> &ps, int which) {
+  const uint32_t w = threadIdx.x / kWave;
+  const uint32_t n = __builtin_amdgcn_readfirstlane(ps.n[w][which]);
+  if (n == 0) return;
+  const uint32_t qi = which ? s.far_q : s.near_q;
+  uint32_t base = 0;
+  if (lane_id() == 0) base = ps.local ? atomicAdd(&ps.tail[which], n) : atomic_add(&a.ctrl->qcount[qi], n);
+  base = readlane(base, 0);
+  for (uint32_t i = lane_id(); i < n; i += kWave) {
+    const uint32_t pos = base + i;
+    if (pos < a.qcap) st_coh(a.q[qi] + pos, ps.item[w][which][i]);
+    else st_coh(&a.ctrl->abort, kAbortCapacity);
+  }
+  ps.n[w][which] = 0;
+}
+
+// Pushes `item` from every lane with `pred` onto pile `which`.
+template <int BS>
+__device__ __forceinline__ void stage_push(const KArgs &a, const Sched &s, PushStage<BS> &ps, int which, bool pred,
+                                           uint32_t item) {
+  const uint64_t m = __ballot(pred);
+  if (m == 0) return;
+  const uint32_t w = threadIdx.x / kWave;
+  const uint32_t c = (uint32_t)__builtin_popcountll(m);
+  uint32_t n = __builtin_amdgcn_readfirstlane(ps.n[w][which]);
+  if (n + c > kStage) {
+    stage_flush<BS>(a, s, ps, which);
+    n = 0;
+  }
+  if (pred) ps.item[w][which][n + lanes_below(m)] = item;
+  ps.n[w][which] = n + c;
+}
+
+// End of a round: every wavefront writes out what it staged.
+template <int BS>
+__device__ __forceinline__ void stage_drain(const KArgs &a, const Sched &s, PushStage<BS> &ps) {
+  stage_flush<BS>(a, s, ps, 0);
+  stage_flush<BS>(a, s, ps, 1);
+}
+
+// Bucket far mode: folds the smallest `x` of the lanes with `pred` into the
+// block's far minimum (one LDS atomic per wavefront).
+template <int BS>
+__device__ __forceinline__ void note_far_min(PushStage<BS> &ps, bool pred, uint32_t x) {
+  if (__ballot(pred) == 0) return;
+  uint32_t m = pred ? x : kInf;
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) m = min(m, (uint32_t)__shfl_xor(m, off));
+  if (lane_id() == 0) atomicMin(&ps.far_min, m);
+}
+
+// End of a round, thread 0 after a block barrier: the block's far minimum
+// goes to the control block (one atomic per block and round).
+template <int BS>
+__device__ __forceinline__ void flush_far_min(const KArgs &a, PushStage<BS> &ps) {
+  if (ps.far_min != kInf) atomic_min(&a.ctrl->far_min, ps.far_min);
+  ps.far_min = kInf;
+}
+
+// ---- per-edge operators ---------------------------------------------------
+
+// Relax u->v with candidate nd = d[u] + w.  Same test and atomicMin as
+// apps/sssp/kernel.cu:129-144, with the push decided by whether the atomicMin
+// really lowered d[v] (the reference pushes on the pre-check alone), and the
+// near/far split at curdelta (:134).  Arithmetic is u32: d[u] < 2^31 and
+// w <= INT_MAX never wrap, and a candidate >= INT_MAX never beats INF.
+struct SsspRelax {
+  template <int BS>
+  __device__ __forceinline__ static void edge(const KArgs &a, const Sched &s, uint32_t v, uint32_t nd, bool active,
+                                              PushStage<BS> &ps, bool precheck) {
+    // The pre-checks read through the cache: a stale d[v] is only ever larger
+    // than the current one (costing at most a redundant atomicMin, whose
+    // return value is authoritative), and a stale stamp is only ever older.
+    // Without `precheck` every edge goes straight to the atomicMin: one
+    // round trip fewer, for rounds too small for the extra atomics to matter.
+    bool imp = false;
+    const bool near = (int64_t)nd <= s.curdelta();
+    uint32_t *stamps = near ? a.near_stamp : a.far_stamp;
+    const uint32_t stamp = near ? s.round + 1 : s.phase + 1;
+    uint32_t seen = stamp;
+    if (active) {
+      const uint32_t dv = precheck ? a.dist[v] : kInf;
+      // the stamp is loaded beside d[v]: one round trip fewer when the edge
+      // improves (C1 12.3 -> 11.9 ms, C3 12.0 -> 11.9 ms, RMAT unchanged)
+      seen = stamps[v];
+      if (!precheck || nd < dv) imp = nd < atomic_min(a.dist + v, nd);
+    }
+    // Push once per round (near) or phase (far): the exchange decides between
+    // racing improvers, the cached pre-check skips it when already stamped.
+    const bool push = imp && seen != stamp && atomic_exch(stamps + v, stamp) != stamp;
+    const bool to_near = push && near, to_far = push && !near;
+    if (s.aux[1] == LSG_FAR_BUCKET) note_far_min<BS>(ps, imp && !near, nd);
+    stage_push<BS>(a, s, ps, 0, to_near, v);
+    stage_push<BS>(a, s, ps, 1, to_far, v);
+  }
+};
+
+// BFS visit.  The reference writes LEVEL when it sees INF and pushes, racing
+// benignly (apps/bfs/kernel.cu:194-200).  Here a level is a unit-weight
+// distance lowered with atomicMin, so a vertex reached late (heavy chunks run
+// one round after their vertex is popped) still ends at its hop count; every
+// improvement is pushed to the next level's pile, once per round.
+struct BfsVisit {
+  template <int BS>
+  __device__ __forceinline__ static void edge(const KArgs &a, const Sched &s, uint32_t v, uint32_t nd, bool active,
+                                              PushStage<BS> &ps, bool precheck) {
+    bool imp = false;
+    if (active) {
+      const uint32_t dv = precheck ? a.dist[v] : kInf;
+      if (!precheck || nd < dv) imp = nd < atomic_min(a.dist + v, nd);
+    }
+    if (imp) {
+      uint32_t stamp = s.round + 1;
+      imp = a.near_stamp[v] != stamp && atomic_exch(a.near_stamp + v, stamp) != stamp;
+    }
+    stage_push<BS>(a, s, ps, 0, imp, v);
+  }
+};
+
+template <Algo A, int BS>
+__device__ __forceinline__ void relax_edge(const KArgs &a, const Sched &s, uint32_t e, uint32_t du, bool active,
+                                           PushStage<BS> &ps, bool precheck) {
+  uint32_t v = 0, nd = 0;
+  if (active) {
+    v = a.edge_dst[e];
+    nd = A == Algo::SSSP ? du + a.edge_data[e] : du + 1u;
+  }
+  if (A == Algo::SSSP) SsspRelax::edge<BS>(a, s, v, nd, active, ps, precheck);
+  else BfsVisit::edge<BS>(a, s, v, nd, active, ps, precheck);
+}
+
+// ---- block-level helpers ---------------------------------------------------
+
+// Exclusive prefix sum over the block; returns this thread's offset and sets
+// *total.  Wave scans with shuffles, wave totals through LDS.
+template <int BS>
+__device__ __forceinline__ uint32_t block_exclusive_sum(uint32_t x, uint32_t *total, uint32_t *wsum) {
+  constexpr int NW = BS / kWave;
+  uint32_t lane = lane_id(), wid = threadIdx.x / kWave;
+  uint32_t incl = x;
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    uint32_t y = __shfl_up(incl, off);
+    if (lane >= (uint32_t)off) incl += y;
+  }
+  if (lane == kWave - 1) wsum[wid] = incl;
+  __syncthreads();
+  if (wid == 0) {
+    uint32_t w = lane < (uint32_t)NW ? wsum[lane] : 0u;
+    uint32_t wi = w;
+#pragma unroll
+    for (int off = 1; off < NW; off <<= 1) {
+      uint32_t y = __shfl_up(wi, off);
+      if (lane >= (uint32_t)off) wi += y;
+    }
+    if (lane < (uint32_t)NW) wsum[lane] = wi - w;  // exclusive wave offsets
+    if (lane == NW - 1) wsum[NW] = wi;
+  }
+  __syncthreads();
+  uint32_t r = wsum[wid] + incl - x;
+  *total = wsum[NW];
+  __syncthreads();
+  return r;
+}
+
+template <int BS>
+struct TileShared {
+  uint32_t offs[BS];
+  uint32_t start[BS];
+  uint32_t du[BS];
+  uint32_t wsum[BS / 64 + 1];
+  unsigned long long edges;  // edges expanded by this block in the current round
+  PushStage<BS> ps;
+};
+
+// One tile of the in-queue: items [t*ti, t*ti+ti), ti <= BS items, one per
+// thread (threads >= ti idle).  Loads the row bounds of
+// each popped vertex (the reference's pop + filter, apps/sssp/kernel.cu:201-210),
+// queues heavy vertices as chunks for the next round, and expands the rest by
+// the selected mode.
+template <Algo A, int BS>
+__device__ void process_tile(const KArgs &a, const Sched &s, uint32_t t, uint32_t ti, TileShared<BS> &sh,
+                             bool precheck = true) {
+  uint32_t i = t * ti + threadIdx.x;
+  bool valid = threadIdx.x < ti && i < s.in_count;
+  uint32_t u = valid ? ld_coh(a.q[s.in_q] + i) : 0u;
+  uint32_t du = 0, rs = 0, re = 0, deg = 0;
+  valid = valid && u < a.nnodes;
+  if (valid) {
+    // distance and row bounds in one round trip
+    du = ld_coh(a.dist + u);
+    rs = a.row_start[u];
+    re = a.row_start[u + 1];
+    if (du == kInf) valid = false;  // INF nodes are never expanded (apps/sssp/kernel.cu:202)
+  }
+  if (A == Algo::SSSP && s.aux[0] && s.aux[1] != LSG_FAR_REFERENCE) {
+    // Split and bucket far piles: an item still above this phase's threshold
+    // re-enters the far pile instead of expanding early.
+    bool defer = valid && (int64_t)du > s.curdelta();
+    if (s.aux[1] == LSG_FAR_BUCKET) note_far_min<BS>(sh.ps, defer, du);
+    if (defer) {
+      uint32_t stamp = s.phase + 1;
+      defer = a.far_stamp[u] != stamp && atomic_exch(a.far_stamp + u, stamp) != stamp;
+      valid = false;
+    }
+    stage_push<BS>(a, s, sh.ps, 1, defer, u);
+  }
+  if (valid) deg = re - rs;
+  // -l: vertices of degree >= lb_limit are the edge-balanced pass's
+  // (apps/sssp/kernel.cu:201: the sweep pops only degrees below DEGREE_LIMIT)
+  if (deg >= a.lb_limit) deg = 0;
+  // Heavy vertices: split into ceil(deg / heavy_deg) chunks, processed by
+  // whole blocks in the next round.
+  bool heavy = deg >= a.heavy_deg;
+  uint32_t nch = heavy ? (deg + a.heavy_deg - 1) / a.heavy_deg : 0u;
+  if (__ballot(heavy)) {
+    uint32_t hq = 1u - s.heavy_in;
+    uint32_t base = wave_reserve(nch, &a.ctrl->hcount[hq]);
+    uint64_t *dst = reinterpret_cast<uint64_t *>(a.hq[hq]);
+    for (uint32_t k = 0; k < nch; ++k) {
+      uint32_t pos = base + k;
+      if (pos < a.hcap) st_coh64(dst + pos, ((uint64_t)k << 32) | u);
+      else st_coh(&a.ctrl->abort, kAbortCapacity);
+    }
+  }
+  if (heavy) deg = 0;
+  if (a.sched || a.feat) {
+    // edges expanded, one of the scheduler's features
+    uint32_t ws = deg;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) ws += __shfl_xor(ws, off);
+    if (lane_id() == 0 && ws) atomicAdd(&sh.edges, (unsigned long long)ws);
+  }
+
+  // Expansion mode of this phase (the learned scheduler may change it).
+  if (s.aux[2] == LSG_EXPAND_LB) {
+    // Tile-flattened expansion: prefix sum of degrees, then every lane takes
+    // one edge of the flattened tile (the reference's fg scheduler,
+    // apps/sssp/kernel.cu:266-302, at block granularity).
+    uint32_t total;
+    uint32_t off = block_exclusive_sum<BS>(deg, &total, sh.wsum);
+    sh.offs[threadIdx.x] = off;
+    sh.start[threadIdx.x] = rs;
+    sh.du[threadIdx.x] = du;
+    __syncthreads();
+    for (uint32_t base = 0; base < total; base += BS) {
+      uint32_t e = base + threadIdx.x;
+      bool act = e < total;
+      uint32_t ee = 0, d0 = 0;
+      if (act) {
+        uint32_t lo = 0, hi = BS;
+        while (hi - lo > 1) {
+          uint32_t mid = (lo + hi) >> 1;
+          if (sh.offs[mid] <= e) lo = mid;
+          else hi = mid;
+        }
+        ee = sh.start[lo] + (e - sh.offs[lo]);
+        d0 = sh.du[lo];
+      }
+      relax_edge<A, BS>(a, s, ee, d0, act, sh.ps, precheck);
+    }
+    __syncthreads();
+  } else {
+    // Degree-binned: one lane per small vertex, the whole wavefront for
+    // medium vertices (the reference's wp scheduler, :218-264, at 64 lanes).
+    bool small = deg < a.thread_deg;
+    uint32_t my = small ? deg : 0u;
+    uint32_t mx = my;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, off));
+    for (uint32_t j = 0; j < mx; ++j) relax_edge<A, BS>(a, s, rs + j, du, j < my, sh.ps, precheck);
+    uint64_t med = __ballot(!small && deg > 0);
+    while (med) {
+      uint32_t l = (uint32_t)__builtin_ctzll(med);
+      med &= med - 1;
+      uint32_t ws = readlane(rs, l), wd = readlane(deg, l), wu = readlane(du, l);
+      for (uint32_t j = 0; j < wd; j += kWave) {
+        uint32_t jj = j + lane_id();
+        relax_edge<A, BS>(a, s, ws + jj, wu, jj < wd, sh.ps, precheck);
+      }
+    }
+  }
+}
+
+// One heavy chunk: heavy_deg consecutive edges of one vertex, spread over
+// the block (replaces the -l edge-balanced TB_LB pass, apps/sssp/kernel.cu:78-149).
+template <Algo A, int BS>
+__device__ void process_heavy(const KArgs &a, const Sched &s, uint32_t c, TileShared<BS> &sh) {
+  uint64_t ent = ld_coh64(reinterpret_cast<const uint64_t *>(a.hq[s.heavy_in]) + c);
+  uint32_t u = (uint32_t)ent, k = (uint32_t)(ent >> 32);
+  if (u >= a.nnodes) return;
+  uint32_t du = ld_coh(a.dist + u);
+  uint32_t rs = a.row_start[u], re = a.row_start[u + 1];
+  if ((uint64_t)k * a.heavy_deg >= re - rs) return;
+  uint32_t lo = rs + k * a.heavy_deg;
+  uint32_t hi = min(re, lo + a.heavy_deg);
+  if ((a.sched || a.feat) && threadIdx.x == 0) atomicAdd(&sh.edges, (unsigned long long)(hi - lo));
+  for (uint32_t b = lo; b < hi; b += BS) {
+    uint32_t e = b + threadIdx.x;
+    relax_edge<A, BS>(a, s, e, du, e < hi, sh.ps, true);
+  }
+}
+
+// Leader step between rounds, run by every thread of the last block to reach
+// the barrier.  Implements the pile rotation of the persistent driver
+// (apps/sssp/kernel.cu:363-384): iterate the near pile to empty (retry2), then
+// make the far pile the next phase's input (advance2) with curdelta += DELTA;
+// finish when both are empty.  At a phase boundary the learned scheduler (when
+// enabled) picks the next phase's delta width and expansion mode; it never
+// changes which vertices are relaxed or how, only when.
+// `s`, `abort_seen` and `far_tail` are the caller's LDS words: on return they
+// hold the schedule just stored, the abort flag loaded with the step's
+// counters and the far pile's count for the next round (its near pile starts
+// empty), so rounds that the same block runs next need not load them again.
+template <Algo A>
+__device__ void leader_step(const KArgs &a, Sched &s, uint32_t &abort_seen, uint32_t &far_tail) {
+  __shared__ SchedFeatures f;
+  __shared__ MlpScratch sc;
+  __shared__ uint32_t decide, consumed, far_min;
+  __shared__ uint64_t phases_seen;
+  Ctrl *c = a.ctrl;
+  if (threadIdx.x == 0) {
+    decide = 0;
+    // Every control word of the step is loaded in one batch: the step costs
+    // one round trip to the coherence point instead of one per dependent load.
+    sched_load(s, &c->s);
+    CtrlCounts k;
+    counts_load(k, c);
+    abort_seen = ld_coh(&c->abort);
+    far_min = k.far_min;
+    const uint32_t nnear = k.q(s.near_q), nfar = k.q(s.far_q), nheavy = k.h(1u - s.heavy_in);
+    far_tail = 0;
+    if (nnear > a.qcap || nfar > a.qcap || nheavy > a.hcap) {
+      // A push found its queue full (and raised kAbortCapacity); never hand
+      // out a count beyond the buffer.
+      st_coh(&c->abort, kAbortCapacity);
+      s.done = 1;
+    } else {
+      st_coh64(&c->stat_items, k.items + s.in_count);
+      st_coh64(&c->stat_heavy, k.heavy + s.heavy_count);
+      st_coh64(&c->stat_rounds, k.rounds + 1);
+      if (a.trace && s.round - 1 < a.trace_cap) {
+        lsg_round_trace &r = a.trace[s.round - 1];
+        r.round = s.round;
+        r.phase = s.phase;
+        r.in_items = s.in_count;
+        r.heavy_chunks = s.heavy_count;
+        r.near_out = nnear;
+        r.far_out = nfar;
+        r.heavy_out = nheavy;
+        r.curdelta_lo = s.curdelta_lo;
+        r.curdelta_hi = s.curdelta_hi;
+        r.delta = s.delta;
+        r.t_end = wall_clock64();
+      }
+      consumed = s.in_q;
+      st_coh(&c->hcount[s.heavy_in], 0u);  // consumed heavy queue becomes the next producer
+      s.heavy_in = 1u - s.heavy_in;
+      s.heavy_count = nheavy;
+      if (nnear > 0 || nheavy > 0) {
+        s.in_q = s.near_q;
+        s.in_count = nnear;
+        s.near_q = consumed;
+        st_coh(&c->qcount[consumed], 0u);
+        s.aux[0] = 0;
+        s.round += 1;
+        far_tail = nfar;
+      } else if (A == Algo::SSSP && nfar > 0) {
+        decide = 1;
+        s.in_count = nfar;
+        f.in_items = nfar;
+        f.phase = s.phase;
+        f.round = s.round;
+        f.curdelta = (float)s.curdelta();
+        f.base_delta = a.delta;
+        f.prev_delta = s.delta;
+        f.lane_items = (float)(k.items + s.in_count);
+        f.edges = (float)k.edges;
+        f.nnodes = a.nnodes;
+        f.nedges = (float)a.nedges;
+        f.nsrc = 1;
+      } else {
+        s.done = 1;
+        st_coh64(&c->stat_phases, k.phases + 1);
+      }
+    }
+    phases_seen = k.phases;
+  }
+  __syncthreads();
+  if (decide) {
+    int act = a.sched ? mlp_block_action(a.mlp, f, sc) : -1;
+    if (threadIdx.x == 0) {
+      if (act >= 0) {
+        s.delta = action_delta(a.mlp, act, a.delta);
+        s.aux[2] = (uint32_t)a.mlp[kOffFarMode + act];  // plain-kernel models: expansion mode
+      }
+      if (a.feat && s.phase < (uint32_t)kMaxForced) {
+        float *rec = a.feat + (size_t)s.phase * kFeatRec;
+        sched_featurize(f, rec);
+        rec[kFeat] = (float)act;
+        rec[kFeat + 1] = (float)s.delta;
+        rec[kFeat + 2] = (float)s.aux[2];
+        rec[kFeat + 3] = (float)(s.round + 1);  // first round of the phase being started
+      }
+      s.in_q = s.far_q;
+      s.far_q = consumed;
+      st_coh(&c->qcount[consumed], 0u);
+      s.aux[0] = 1;  // this round pops the far pile
+      // apps/sssp/kernel.cu:383: curdelta += DELTA.  In bucket mode the
+      // threshold moves on by whole steps of delta until it reaches the
+      // smallest distance waiting in the far pile (delta-stepping's lowest
+      // non-empty bucket), so empty buckets cost no phase.
+      int64_t cd = s.curdelta() + s.delta;
+      if (s.aux[1] == LSG_FAR_BUCKET) {
+        if (far_min != kInf && (int64_t)far_min > cd) cd += ((int64_t)far_min - cd + s.delta - 1) / s.delta * s.delta;
+        st_coh(&c->far_min, kInf);
+      }
+      s.set_curdelta(cd);
+      s.phase += 1;
+      s.round += 1;
+      st_coh64(&c->stat_phases, phases_seen + 1);
+    }
+  }
+  if (threadIdx.x == 0) {
+    sched_store(&c->s, s);
+    st_coh(&c->ticket, 0u);
+  }
+}
+
+template <int BS>
+__device__ __forceinline__ uint32_t tile_items(uint32_t in_count, uint32_t nblocks) {
+  uint32_t per = (in_count + nblocks - 1) / nblocks;
+  per = (per + kWave - 1) / kWave * kWave;
+  return per < (uint32_t)kWave ? (uint32_t)kWave : per > (uint32_t)BS ? (uint32_t)BS : per;
+}
+
+// Rounds small enough for one block.  While the next round's in-queue fits in
+// kLocalTiles tiles and has no heavy chunks, the block that ran the leader
+// step runs that round itself, separated by block barriers, while the rest of
+// the grid stays parked at the grid barrier; it bumps `progress` each round so
+// the parked blocks' spin limit measures lack of progress.  A round of a small
+// frontier (high-diameter graphs, the first and last rounds of any traversal)
+// then costs a few dependent memory round trips instead of a grid barrier.
+// One tile per local round measured fastest (1, 2, 4 tiles x blocks of 256,
+// 512, 1024 threads: 12.4-16.4 ms on the 256^2 grid, one tile of 256 best).
+// Local rounds skip the d[v] pre-check and go straight to the atomicMin (one
+// round trip fewer; for a few hundred edges the extra atomics cost less).
+constexpr uint32_t kLocalTiles = 1;
+
+// `ls` and `labort`: the schedule and abort flag the leader step just left in LDS.
+template <Algo A, int BS>
+__device__ void local_rounds(const KArgs &a, TileShared<BS> &sh, Sched &ls, uint32_t &labort, uint32_t &lfar) {
+  __shared__ uint32_t go;
+  Ctrl *c = a.ctrl;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      go = !ls.done && ls.heavy_count == 0 && ls.in_count <= kLocalTiles * BS && labort == kAbortNone;
+      sh.edges = 0;
+      sh.ps.local = go;
+      sh.ps.tail[0] = 0;  // the near pile starts empty
+      sh.ps.tail[1] = lfar;
+    }
+    __syncthreads();
+    if (!go) return;
+    const Sched s = ls;
+    for (uint32_t t = 0; t * BS < s.in_count; ++t) {
+      process_tile<A, BS>(a, s, t, BS, sh, false);
+      __syncthreads();
+    }
+    stage_drain<BS>(a, s, sh.ps);
+    __syncthreads();  // every wavefront has taken its places from the LDS tails
+    if (threadIdx.x == 0) {
+      st_coh(&c->qcount[s.near_q], sh.ps.tail[0]);
+      st_coh(&c->qcount[s.far_q], sh.ps.tail[1]);
+      flush_far_min<BS>(a, sh.ps);
+      if ((a.sched || a.feat) && sh.edges)
+        __hip_atomic_fetch_add(&c->stat_edges, (uint64_t)sh.edges, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    // this block's pushes and counts are complete before the leader step reads them
+    order_release();
+    __syncthreads();
+    leader_step<A>(a, ls, labort, lfar);
+    if (threadIdx.x == 0) st_coh(&c->progress, s.round);
+  }
+}
+
+template <Algo A, int BS>
+__global__ __launch_bounds__(BS) void persistent_kernel(KArgs a) {
+  __shared__ TileShared<BS> sh;
+  __shared__ Sched ss;
+  __shared__ Sched lsched;  // the leader step's schedule, reused by local rounds
+  __shared__ uint32_t labort, lfar;
+  __shared__ uint32_t tk;
+  __shared__ uint32_t quit;
+  Ctrl *c = a.ctrl;
+  const uint32_t nb = gridDim.x;
+  if (!residency_handshake(c, nb, a.residency_limit)) return;
+  if (threadIdx.x < 2 * (BS / kWave)) (&sh.ps.n[0][0])[threadIdx.x] = 0;
+  if (threadIdx.x == 0) {
+    sh.ps.local = 0;
+    sh.ps.far_min = kInf;
+  }
+  for (;;) {
+    if (threadIdx.x == 0) {
+      Sched s;
+      sched_load(s, &c->s);
+      ss = s;
+      quit = s.done || ld_coh(&c->abort) != kAbortNone;
+      sh.edges = 0;
+    }
+    __syncthreads();
+    if (quit) break;
+    const Sched s = ss;
+    // Tile size: BS items, or fewer when that leaves blocks idle -- a small
+    // frontier spread over more blocks finishes its round in fewer dependent
+    // steps (multiples of 64 items, one wavefront's worth, at least).
+    const uint32_t ti = tile_items<BS>(s.in_count, nb);
+    const uint32_t ntiles = (s.in_count + ti - 1) / ti;
+    const uint32_t total = ntiles + s.heavy_count;
+    // Static first pass (ticket = block id), then dynamic tickets.
+    uint32_t t = blockIdx.x;
+    while (t < total) {
+      if (t < ntiles) process_tile<A, BS>(a, s, t, ti, sh);
+      else process_heavy<A, BS>(a, s, t - ntiles, sh);
+      if (total <= nb) break;
+      __syncthreads();
+      if (threadIdx.x == 0) tk = nb + atomic_add(&c->ticket, 1u);
+      __syncthreads();
+      t = tk;
+    }
+    stage_drain<BS>(a, s, sh.ps);
+    if (a.sched || a.feat || s.aux[1] == LSG_FAR_BUCKET) {
+      __syncthreads();
+      if (threadIdx.x == 0 && sh.edges)
+        __hip_atomic_fetch_add(&c->stat_edges, (uint64_t)sh.edges, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (threadIdx.x == 0) flush_far_min<BS>(a, sh.ps);
+    }
+    grid_barrier_block(c, nb, a.spin_limit, [&] {
+      leader_step<A>(a, lsched, labort, lfar);
+      local_rounds<A, BS>(a, sh, lsched, labort, lfar);
+    });
+  }
+}
+
+// Initialise distances, stamps and the control block (apps/sssp/kernel.cu:23-35,469;
+// apps/bfs/kernel.cu:20-32).  The sources are seeded by seed_kernel.
+static __global__ void init_kernel(KArgs a, uint32_t delta) {
+  uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
+  for (uint32_t v = tid; v < a.nnodes; v += nt) {
+    a.dist[v] = kInf;
+    a.near_stamp[v] = 0u;
+    a.far_stamp[v] = 0u;
+  }
+  ctrl_reset_barrier(a.ctrl, tid, nt);
+  if (tid == 0) {
+    Ctrl *c = a.ctrl;
+    c->qcount[0] = 0;
+    c->qcount[1] = 0;
+    c->qcount[2] = 0;
+    c->hcount[0] = 0;
+    c->hcount[1] = 0;
+    c->ticket = 0;
+    c->abort = kAbortNone;
+    c->checkin = 0;
+    c->resident_seen = 0;
+    c->far_min = kInf;
+    Sched s;
+    memset(&s, 0, sizeof s);
+    s.in_q = 0;
+    s.near_q = 1;
+    s.far_q = 2;
+    s.round = 1;
+    s.set_curdelta(0);  // apps/sssp/kernel.cu:472
+    s.delta = delta;
+    s.aux[1] = a.far_mode;
+    s.aux[2] = a.expand_mode;
+    c->s = s;
+    c->stat_items = c->stat_rounds = c->stat_heavy = c->stat_phases = c->stat_edges = 0;
+  }
+}
+
+// d[seed] = 0 and the seeds form the first in-queue (apps/sssp/kernel.cu:33,476-477).
+// One block.  Several seeds make a multi-seed traversal: on a disjoint union of
+// graphs with one seed each it computes every graph's own distances.
+static __global__ void seed_kernel(KArgs a, const uint32_t *seeds, uint32_t nseeds) {
+  __shared__ uint32_t cnt;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < nseeds; i += blockDim.x) {
+    uint32_t v = seeds[i];
+    if (v < a.nnodes && atomicExch(a.dist + v, 0u) != 0u) a.q[0][atomicAdd(&cnt, 1u)] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    a.ctrl->qcount[0] = cnt;
+    a.ctrl->s.in_count = cnt;
+  }
+}
+
+// ---- host helpers shared by the persistent and the -l launches ----------
+
+// The arrays of one traversal: a device CSR, possibly a packed union of graphs.
+struct CsrView {
+  const uint32_t *row, *dst, *w;
+  uint32_t nnodes;
+  uint64_t nedges;
+};
+
+inline int check_single_opts(const lsg_opts &o, bool sssp, int *bs, int *expand) {
+  if (sssp && o.delta <= 0) return fail(LSG_ERR_ARG, "delta must be > 0 (got %d)", o.delta);
+  *bs = o.block_size ? o.block_size : 256;
+  if (*bs != 256 && *bs != 512 && *bs != 1024) return fail(LSG_ERR_ARG, "block_size must be 256, 512 or 1024");
+  *expand = o.expand_mode;
+  if (*expand != LSG_EXPAND_TWB && *expand != LSG_EXPAND_LB) return fail(LSG_ERR_ARG, "bad expand_mode %d", *expand);
+  if (o.far_mode != LSG_FAR_REFERENCE && o.far_mode != LSG_FAR_SPLIT && o.far_mode != LSG_FAR_BUCKET)
+    return fail(LSG_ERR_ARG, "bad far_mode %d", o.far_mode);
+  if (o.lb_deg < 0) return fail(LSG_ERR_ARG, "lb_deg must be >= 0 (got %d)", o.lb_deg);
+  if (o.sched && !sssp) return fail(LSG_ERR_ARG, "the learned scheduler chooses SSSP delta widths; BFS has none");
+  return LSG_OK;
+}
+
+// Validates the options of a single-source run and fills the kernel
+// arguments: workspace buffers (queues, heavy queues, stamps, the seed list
+// of `nseeds` entries), schedule knobs, scheduler model, trace buffers.
+template <Algo A>
+inline int setup_single(Workspace *w, const CsrView &g, uint32_t nseeds, const lsg_opts &o, uint32_t *dist, KArgs *ap,
+                        void **seedbuf_out, int *bs_out) {
+  int bs, expand, rc;
+  if ((rc = check_single_opts(o, A == Algo::SSSP, &bs, &expand))) return rc;
+  uint32_t thread_deg = o.thread_deg > 0 ? (uint32_t)o.thread_deg : 8u;
+  // 512: vertices of a few hundred edges and up are spread over the grid as
+  // chunks (RMAT-20 from the hub: 10.0 ms at 4096, 5.8 ms at 512; 128-1024 alike).
+  uint32_t heavy_deg = o.heavy_deg > 0 ? (uint32_t)o.heavy_deg : 512u;
+  if (heavy_deg < (uint32_t)bs) heavy_deg = (uint32_t)bs;
+  uint64_t hcap64 = 2ull * g.nedges / heavy_deg + 64;
+  uint32_t hcap = (uint32_t)std::min<uint64_t>(hcap64, UINT32_MAX / 2);
+  void *qbuf, *hbuf, *sbuf, *seedbuf;
+  if ((rc = ws_buffer(w, kBufQueues, 3ull * g.nnodes * 4, &qbuf)) ||
+      (rc = ws_buffer(w, kBufHeavy, 2ull * hcap * 8, &hbuf)) ||
+      (rc = ws_buffer(w, kBufStamps, 2ull * g.nnodes * 4, &sbuf)) ||
+      (rc = ws_buffer(w, kBufSeedList, (size_t)nseeds * 4, &seedbuf)))
+    return rc;
+
+  KArgs &a = *ap;
+  memset(&a, 0, sizeof a);
+  a.row_start = g.row;
+  a.edge_dst = g.dst;
+  a.edge_data = g.w;
+  a.dist = dist;
+  for (int i = 0; i < 3; ++i) a.q[i] = (uint32_t *)qbuf + (size_t)i * g.nnodes;
+  a.hq[0] = (uint32_t *)hbuf;
+  a.hq[1] = (uint32_t *)hbuf + 2ull * hcap;
+  a.near_stamp = (uint32_t *)sbuf;
+  a.far_stamp = (uint32_t *)sbuf + g.nnodes;
+  a.ctrl = w->ctrl;
+  a.nnodes = g.nnodes;
+  a.qcap = g.nnodes;
+  a.hcap = hcap;
+  a.delta = A == Algo::SSSP ? (uint32_t)o.delta : 1u;
+  a.thread_deg = thread_deg;
+  a.heavy_deg = heavy_deg;
+  a.expand_mode = (uint32_t)expand;
+  a.far_mode = (uint32_t)o.far_mode;
+  a.spin_limit = spin_limit_ticks(w);
+  a.residency_limit = residency_limit_ticks(w);
+  a.nedges = g.nedges;
+  if (o.sched) {
+    a.sched = 1;
+    a.mlp = sched_model_device(w->device, LSG_SCHED_PLAIN);
+    if (!a.mlp)
+      return fail(LSG_ERR_ARG, "learned scheduler requested but no plain-kernel model is loaded (lsg_sched_set_model)");
+  }
+  if (o.trace) {
+    void *tbuf;
+    const size_t tbytes = kTraceCap * sizeof(lsg_round_trace);
+    if ((rc = ws_buffer(w, kBufTrace, tbytes + kMaxForced * kFeatRec * sizeof(float), &tbuf))) return rc;
+    a.trace = (lsg_round_trace *)tbuf;
+    a.trace_cap = kTraceCap;
+    if (A == Algo::SSSP) a.feat = (float *)((char *)tbuf + tbytes);
+  }
+  a.lb_limit = UINT32_MAX;
+  *seedbuf_out = seedbuf;
+  *bs_out = bs;
+  return LSG_OK;
+}
+
+// The -l loop (lb.hip) for algo = (int)Algo::SSSP or Algo::BFS.
+int run_lb_single(int algo, Workspace *w, const CsrView &g, uint32_t seed, const lsg_opts &o, uint32_t *dist,
+                  lsg_stats *st);
+
+}  // namespace lsg

@@ -67,9 +67,11 @@ enum {
 enum {
   LSG_FAR_REFERENCE = 0, /* whole far pile becomes the next phase (apps/sssp/kernel.cu:374-383) */
   LSG_FAR_SPLIT = 1,     /* far items above the next threshold wait for a later phase */
-  LSG_FAR_BUCKET = 2     /* batched only: each source's threshold is its own smallest pending
-                            distance + delta - 1 (delta-stepping's lowest non-empty bucket), items
-                            above it wait */
+  LSG_FAR_BUCKET = 2     /* delta-stepping's lowest non-empty bucket: items above the threshold
+                            wait as in SPLIT, and the threshold skips empty buckets.  Single
+                            source: curdelta moves on by whole steps of delta until it reaches
+                            the smallest waiting distance.  Batched: each source's threshold is
+                            its own smallest pending distance + delta - 1 */
 };
 
 /* Options of the sssp/bfs apps: -d, -s and -l (apps/sssp/support.cu:6-31,
@@ -78,7 +80,11 @@ enum {
 typedef struct lsg_opts {
   int32_t delta;       /* -d, sssp only; default 10000 (apps/sssp/support.cu:10) */
   int32_t start_node;  /* -s; default 0 (apps/sssp/support.cu:12) */
-  int32_t enable_lb;   /* -l: heavy vertices (degree >= total threads) are edge-balanced */
+  int32_t enable_lb;   /* -l: the reference's host-driven loop with an edge-balanced pass
+                          (apps/sssp/kernel.cu:318-354, apps/bfs/kernel.cu:286-311): each round
+                          an inspect pass lists the frontier vertices of degree >= lb_deg, a prefix
+                          sum of their degrees spreads their edges one per thread over the grid,
+                          and the sweep relaxes the other vertices */
   int32_t expand_mode; /* LSG_EXPAND_* */
   int32_t far_mode;    /* LSG_FAR_* */
   int32_t block_size;  /* threads per block: 256 (default), 512 or 1024 */
@@ -92,7 +98,9 @@ typedef struct lsg_opts {
   int32_t trace;       /* 1 = record one lsg_round_trace per round (lsg_last_trace) */
   int32_t graph_launch; /* 1 = launch the kernel sequence as a captured hipGraph (replayed while
                            the arguments stay the same) */
-  int32_t reserved[4];
+  int32_t lb_deg;      /* -l: degree threshold of the edge-balanced pass; 0 = the sweep's total
+                          thread count, the reference's DEGREE_LIMIT (rt/include/ggcuda.h:19-22) */
+  int32_t reserved[3];
 } lsg_opts;
 
 /* One record per round of a single-source traversal, written when
@@ -154,9 +162,10 @@ int lsg_graph_copy_to_cpu(const lsg_graph *dev, lsg_graph *host);
 /* CSRGraph::dealloc (rt/src/csr_graph.cu:243-248). */
 void lsg_graph_dealloc(lsg_graph *g);
 /* Reverse CSR of a device graph, built on its device: gt's row v lists the
- * sources of v's in-edges (in no particular order) with their weights.  gt
- * is allocated here and freed with lsg_graph_dealloc.  No reference
- * counterpart; bottom-up BFS steps read it. */
+ * sources of v's in-edges with their weights, in forward order (ascending
+ * source, and for one source in the order of its out-edges), the same on
+ * every call.  gt is allocated here and freed with lsg_graph_dealloc.  No
+ * reference counterpart; bottom-up BFS steps and PageRank read it. */
 int lsg_graph_transpose(const lsg_graph *dev, lsg_graph *gt);
 /* Wraps caller-owned device arrays (for example torch tensors) as a device
  * graph without copying.  node_data may be NULL when only *_dist entry points

@@ -45,6 +45,10 @@ MODES = [
     dict(block_size=1024, expand_mode=1),
     dict(heavy_deg=256, thread_deg=2),
     dict(heavy_deg=256, expand_mode=1, grid_blocks=7),
+    dict(far_mode=1),
+    dict(far_mode=2, expand_mode=0),
+    dict(far_mode=2, block_size=512, heavy_deg=512),
+    dict(enable_lb=1, lb_deg=3),
 ]
 
 
@@ -88,7 +92,8 @@ def test_rmat20_config2_sssp(rmat20):
     src = g.max_degree_vertex()
     ref = oracle.dijkstra(g.row_start, g.edge_dst, g.edge_data, src)
     dev = g.copy_to_gpu()
-    for mode in (dict(), dict(expand_mode=1), dict(block_size=512, heavy_deg=1024)):
+    for mode in (dict(), dict(expand_mode=1), dict(block_size=512, heavy_deg=1024), dict(far_mode=1),
+                 dict(far_mode=2), dict(far_mode=2, expand_mode=0, delta=7)):
         say("rmat20 sssp", mode)
         d = run_sssp(g, dev, src, 32, **mode)
         assert np.array_equal(d, ref), mode
