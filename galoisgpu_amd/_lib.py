@@ -182,6 +182,7 @@ SIGNATURES = {
     "lsg_mlp_train": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_uint32, c_uint32, c_float,
                               c_float, c_uint64, c_void_p]),
     "lsg_mlp_infer": (c_int, [c_int, c_void_p, c_void_p, c_uint32, c_void_p]),
+    "lsg_workspace_release": (c_int, [c_int]),
     "lsg_dev_malloc": (c_int, [c_int, c_size_t, POINTER(c_void_p)]),
     "lsg_dev_free": (c_int, [c_int, c_void_p]),
     "lsg_dev_copy": (c_int, [c_int, c_void_p, c_void_p, c_size_t, c_int]),

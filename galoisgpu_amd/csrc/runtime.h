@@ -30,6 +30,7 @@ enum BufSlot {
   kBufSeedList,
   kBufUnion,
   kBufLb,
+  kBufValidate,
   kBufCount
 };
 
@@ -102,7 +103,7 @@ int get_workspace(int device, Workspace **out);
 int ws_buffer(Workspace *w, int slot, size_t bytes, void **out);
 // Grid barrier spin bound (LSG_BARRIER_TIMEOUT_S, default 20 s) in wall-clock ticks.
 uint64_t spin_limit_ticks(const Workspace *w);
-// Start-up residency handshake bound (50 ms) in wall-clock ticks.
+// Start-up residency handshake bound (5 ms) in wall-clock ticks.
 uint64_t residency_limit_ticks(const Workspace *w);
 // Largest grid believed co-resident for (kernel, block size).
 int resident_grid(Workspace *w, const void *fn, int bs, int *grid);

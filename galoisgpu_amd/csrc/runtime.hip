@@ -131,6 +131,23 @@ int lsg_stream_copy(int device, size_t bytes, int iters, double *ms_per_copy) {
   return LSG_OK;
 }
 
+int lsg_workspace_release(int device) {
+  Workspace *w;
+  int rc = get_workspace(device, &w);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(w->mu);
+  HIPCHK(hipSetDevice(device));
+  HIPCHK(hipStreamSynchronize(w->stream));
+  for (auto &c : w->captured) (void)hipGraphExecDestroy(c.exec);
+  w->captured.clear();
+  for (int i = 0; i < kBufCount; ++i) {
+    if (w->buf[i]) HIPCHK(hipFree(w->buf[i]));
+    w->buf[i] = nullptr;
+    w->buf_bytes[i] = 0;
+  }
+  return LSG_OK;
+}
+
 int lsg_dev_malloc(int device, size_t bytes, void **ptr) {
   if (!ptr) return fail(LSG_ERR_ARG, "dev_malloc: NULL out pointer");
   *ptr = nullptr;

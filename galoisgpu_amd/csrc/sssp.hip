@@ -216,6 +216,22 @@ __global__ void unpack_union_kernel(const UnionMember *m, uint32_t n, const uint
   }
 }
 
+// Checks a device CSR in place: row_start[0] == 0, offsets never decrease,
+// row_start[V] == E, every destination < V, every weight >= 0.  bad[0] gets
+// the first failing check (1 offsets, 2 destination, 3 weight) -- any one, by
+// atomicMin over (check, index) -- and bad[1] its index.
+__global__ void validate_csr_kernel(const uint32_t *row, const uint32_t *dst, const int32_t *w, uint32_t V,
+                                    uint32_t E, unsigned long long *bad) {
+  const size_t tid = blockIdx.x * (size_t)blockDim.x + threadIdx.x, nt = (size_t)gridDim.x * blockDim.x;
+  if (tid == 0 && (row[0] != 0 || row[V] != E)) atomicMin(bad, 1ull << 32);
+  for (size_t i = tid; i < V; i += nt)
+    if (row[i + 1] < row[i]) atomicMin(bad, (1ull << 32) | i);
+  for (size_t e = tid; e < E; e += nt) {
+    if (dst[e] >= V) atomicMin(bad, (2ull << 32) | e);
+    else if (w && w[e] < 0) atomicMin(bad, (3ull << 32) | e);
+  }
+}
+
 }  // namespace lsg
 
 using namespace lsg;
@@ -301,8 +317,35 @@ void lsg_graph_dealloc(lsg_graph *g) {
 
 int lsg_graph_wrap_device(lsg_graph *d, uint32_t nnodes, uint32_t nedges, uint32_t *row_start, uint32_t *edge_dst,
                           int32_t *edge_data, int32_t *node_data, int device) {
-  if (!d || !row_start || (nedges && (!edge_dst || !edge_data)) || nnodes == 0)
+  clear_error();
+  if (!d || !row_start || (nedges && (!edge_dst || !edge_data)) || nnodes == 0 || nnodes == UINT32_MAX)
     return fail(LSG_ERR_ARG, "wrap_device: bad arguments");
+  // The kernels trust the CSR (an out-of-range destination would make their
+  // atomics write out of bounds), so it is checked on the device first.
+  Workspace *w;
+  int rc = get_workspace(device, &w);
+  if (rc) return rc;
+  {
+    std::lock_guard<std::mutex> lk(w->mu);
+    HIPCHK(hipSetDevice(device));
+    void *vb;
+    if ((rc = ws_buffer(w, kBufValidate, 256, &vb))) return rc;
+    unsigned long long *bad = (unsigned long long *)vb;
+    const unsigned long long none = ~0ull;
+    HIPCHK(hipMemcpyAsync(bad, &none, 8, hipMemcpyHostToDevice, w->stream));
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>(((uint64_t)nnodes + nedges + 255) / 256 + 1, 16ull * w->cus);
+    hipLaunchKernelGGL(validate_csr_kernel, dim3(blocks), dim3(256), 0, w->stream, (const uint32_t *)row_start,
+                       (const uint32_t *)edge_dst, (const int32_t *)edge_data, nnodes, nedges, bad);
+    HIPCHK(hipGetLastError());
+    unsigned long long hb = 0;
+    HIPCHK(hipMemcpyAsync(&hb, bad, 8, hipMemcpyDeviceToHost, w->stream));
+    HIPCHK(hipStreamSynchronize(w->stream));
+    const uint32_t what = (uint32_t)(hb >> 32), at = (uint32_t)hb;
+    if (what == 1) return fail(LSG_ERR_FORMAT, "wrap_device: row offsets are not a CSR (row_start[0] = 0, "
+                                               "non-decreasing, row_start[%u] = %u) near node %u", nnodes, nedges, at);
+    if (what == 2) return fail(LSG_ERR_FORMAT, "wrap_device: edge %u has destination >= %u", at, nnodes);
+    if (what == 3) return fail(LSG_ERR_FORMAT, "wrap_device: edge %u has a negative weight", at);
+  }
   memset(d, 0, sizeof *d);
   d->nnodes = nnodes;
   d->nedges = nedges;

@@ -118,10 +118,15 @@ typedef struct lsg_round_trace {
   uint64_t t_end;        /* device wall clock at the end of the round */
 } lsg_round_trace;
 
-/* Run statistics.  ms_total mirrors the reference's timed region around
- * gg_main (skelapp/skel.cu:60-63), which includes worklist allocation
- * (apps/sssp/kernel.cu:459,475); ms_kernel is device time of the traversal
- * kernels only (HIP events on the library's stream). */
+/* Run statistics.  ms_total is host time of the whole call, the reference's
+ * timed region around gg_main (skelapp/skel.cu:60-63).  The reference
+ * allocates its worklists inside that region on every call
+ * (apps/sssp/kernel.cu:459,475); this library keeps its device buffers
+ * between calls, so ms_total includes allocation only on a device's first
+ * call or after lsg_workspace_release.  ms_kernel is device time of the
+ * traversal kernels only (HIP events on the library's stream; for -l runs,
+ * the whole host-driven loop).  heavy_chunks: heavy chunks consumed, or for
+ * -l runs the edges relaxed by the edge-balanced pass. */
 typedef struct lsg_stats {
   double ms_total;
   double ms_kernel;
@@ -303,6 +308,13 @@ int lsg_mlp_infer(int device, const float *model, const float *X, uint32_t n, in
 int lsg_output(const lsg_graph *g, const char *path);
 /* The same text for an explicit distance array. */
 int lsg_output_dist(const int32_t *dist, uint32_t n, const char *path);
+
+/* Frees the device buffers the traversals of `device` keep between calls
+ * (queues, stamps, masks, batched distance planes, captured hipGraphs); the
+ * next call allocates them again.  The reference allocates its worklists
+ * inside every gg_main (apps/sssp/kernel.cu:459,475): a call made right after
+ * this one is timed the way the reference's is. */
+int lsg_workspace_release(int device);
 
 /* Device memory for callers without a GPU runtime of their own (the Python
  * host side, tests, benchmarks).  kind: 0 host->device, 1 device->host,

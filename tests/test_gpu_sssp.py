@@ -280,3 +280,61 @@ def test_bmk_harness_runs_the_cli(tmp_path):
     apps.output(ref, bad)
     r = bmk.run("sssp", path, delta=32, source=5, ref=bad)
     assert r["check"] == "fail" and r["line"] == 8
+
+
+def test_wrap_device_validates_the_csr():
+    """lsg_graph_wrap_device checks caller-owned device arrays on the device: an edge out of
+    range, decreasing offsets, a row_start that does not end at nedges and a negative weight
+    are LSG_ERR_FORMAT; a good CSR runs and gives the oracle's distances."""
+    from galoisgpu_amd.batch import DeviceArray
+    from galoisgpu_amd.graph import DeviceGraph
+    g = CSRGraph.rmat(12, 8, seed=9)
+    row, dst, w = g.row_start.copy(), g.edge_dst.copy(), g.edge_data.copy()
+    V, E = g.nnodes, g.nedges
+
+    def wrap(r, d, x):
+        bufs = [DeviceArray.from_numpy(np.ascontiguousarray(a)) for a in (r, d, x)]
+        try:
+            return DeviceGraph.wrap(V, E, bufs[0].ptr, bufs[1].ptr, bufs[2].ptr), bufs
+        except LsgError:
+            for b in bufs:
+                b.close()
+            raise
+
+    bad_dst = dst.copy()
+    bad_dst[E // 2] = V
+    bad_row = row.copy()
+    k = V // 2
+    bad_row[k] = row[k + 1] + 1  # offsets decrease from k to k + 1
+    end_row = row.copy()
+    end_row[V] = E - 1
+    neg = w.copy()
+    neg[E - 1] = -5
+    for name, r, d, x in (("dst", row, bad_dst, w), ("offsets", bad_row, dst, w), ("end", end_row, dst, w),
+                          ("weight", row, dst, neg)):
+        say("wrap rejects", name)
+        with pytest.raises(LsgError) as e:
+            wrap(r, d, x)
+        assert e.value.name == "LSG_ERR_FORMAT", name
+    dg, bufs = wrap(row, dst, w)
+    out = DeviceArray(V * 4)
+    src = g.max_degree_vertex()
+    apps.sssp_into(dg, out.ptr, Options(delta=32, start_node=src))
+    assert np.array_equal(out.to_numpy(np.int32, V), oracle.dijkstra(row, dst, w, src))
+    for b in bufs + [out]:
+        b.close()
+
+
+def test_workspace_release_then_allocating_call():
+    """After lsg_workspace_release the next call allocates its buffers again and still gives
+    the oracle's distances (the with-allocation timing of the reference's gg_main)."""
+    from galoisgpu_amd._lib import check, lib
+    g = CSRGraph.rmat(14, 16, seed=11)
+    src = g.max_degree_vertex()
+    ref = oracle.dijkstra(g.row_start, g.edge_dst, g.edge_data, src)
+    dev = g.copy_to_gpu()
+    for rep in range(2):
+        check(lib().lsg_workspace_release(0), "release")
+        assert np.array_equal(run_sssp(g, dev, src, 32), ref), rep
+        assert np.array_equal(run_sssp(g, dev, src, 32, enable_lb=1, lb_deg=64), ref), rep
+    dev.close()
