@@ -165,7 +165,7 @@ __device__ __forceinline__ uint64_t bcast64(uint64_t x, uint32_t l) {
 // once (a pop that took its next-round bits leaves it queued while its mask
 // is empty again).
 __device__ __forceinline__ bool near_admit(const MArgs &a, const Sched &s, uint32_t v) {
-  return __hip_atomic_fetch_max(a.qstamp + v, s.round + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < s.round + 1;
+  return __hip_atomic_fetch_max(glb(a.qstamp + v), s.round + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < s.round + 1;
 }
 
 // Distance rows of the items a wavefront is expanding: dl[slot][lane] is the
@@ -363,7 +363,7 @@ __device__ __forceinline__ void relax_edges(const MArgs &a, const Sched &s, Lane
   bool have_prev = false;
   for (;;) {
     window_load(a, ls, du, vv, ww, e - w0, needA, needB, cur);
-    cur.dst = lane < kGroup ? a.edge_dst[w0 + lane] : 0u;
+    cur.dst = lane < kGroup ? glb(a.edge_dst)[w0 + lane] : 0u;
     const uint32_t en = e + kGroup;
     const bool more = en < hi;
     const uint32_t wn = more ? min(en, hi - kGroup) : w0;
@@ -421,8 +421,8 @@ __device__ __forceinline__ void process_items(const MArgs &a, const Sched &s, La
     // Gauss-Seidel pop: the bits set for the next round so far are expanded
     // now, with the distances that set them; u stays queued for the next round.
     m |= atomic_exch64(a.nmask[s.round & 1] + u, 0ull);
-    rs = a.row_start[u];
-    re = a.row_start[u + 1];
+    rs = glb(a.row_start)[u];
+    re = glb(a.row_start)[u + 1];
   }
   // The row loads depend on the exchanged masks, so each reads a distance at
   // least as new as the improvements that set the consumed bits.
@@ -488,8 +488,8 @@ __device__ __forceinline__ void process_items(const MArgs &a, const Sched &s, La
     const uint32_t e = __shfl(rs, item) + (f - __shfl(first, item));
     uint32_t ev = 0, ew = 0;
     if (f < total) {
-      ev = a.edge_dst[e];
-      ew = a.edge_data[e];
+      ev = glb(a.edge_dst)[e];
+      ew = glb(a.edge_data)[e];
     }
     relax_batch(a, s, ls, dl, min((uint32_t)kWave, total - b0), ev, ew, item, &mc.improved);
   }
@@ -508,7 +508,7 @@ __device__ __forceinline__ void process_chunk(const MArgs &a, const Sched &s, La
   uint64_t m = ld_coh64(a.hq[s.heavy_in] + 2ull * c + 1);
   uint32_t u = (uint32_t)ent, k = (uint32_t)(ent >> 32);
   if (u >= a.nnodes) return;
-  uint32_t rs = a.row_start[u], re = a.row_start[u + 1];
+  uint32_t rs = glb(a.row_start)[u], re = glb(a.row_start)[u + 1];
   if ((uint64_t)k * a.heavy_deg >= re - rs) return;
   uint32_t lo = rs + k * a.heavy_deg, hi = min(re, lo + a.heavy_deg);
   bool act = lane < a.S && ((m >> lane) & 1);

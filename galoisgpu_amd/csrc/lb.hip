@@ -62,7 +62,7 @@ __global__ __launch_bounds__(kLbBlock) void lb_inspect_kernel(KArgs a, LbLists l
     if (i < s.in_count) {
       u = ld_coh(a.q[s.in_q] + i);
       if (u < a.nnodes) {
-        deg = a.row_start[u + 1] - a.row_start[u];
+        deg = glb(a.row_start)[u + 1] - glb(a.row_start)[u];
         pick = deg >= a.lb_limit;
         if (pick && A == Algo::SSSP) {
           const uint32_t du = ld_coh(a.dist + u);
@@ -77,8 +77,8 @@ __global__ __launch_bounds__(kLbBlock) void lb_inspect_kernel(KArgs a, LbLists l
     base = readlane(base, (uint32_t)__builtin_ctzll(m));
     if (pick) {
       const uint32_t pos = base + lanes_below(m);
-      l.deg[pos] = deg;
-      l.src[pos] = u;
+      glb(l.deg)[pos] = deg;
+      glb(l.src)[pos] = u;
     }
   }
 }
@@ -86,7 +86,7 @@ __global__ __launch_bounds__(kLbBlock) void lb_inspect_kernel(KArgs a, LbLists l
 // Inclusive prefix from the tile scan: prefix[i] = exclusive[i] + tile offset + deg[i].
 __global__ void lb_prefix_kernel(LbLists l, uint32_t n) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    l.prefix[i] += l.tiles[i / kScanTile] + l.deg[i];
+    glb(l.prefix)[i] += glb(l.tiles)[i / kScanTile] + glb(l.deg)[i];
 }
 
 // compute_src_and_offset (rt/include/thread_work.h:51-79): the list entry
@@ -115,11 +115,12 @@ __global__ __launch_bounds__(kLbBlock) void lb_relax_kernel(KArgs a, LbLists l) 
     sched_load(ss, &a.ctrl->s);
     sh.ps.local = 0;
     sh.ps.far_min = kInf;
+    sh.ps.near_lds = sh.ps.in_lds = 0;
   }
   __syncthreads();
   const Sched s = ss;
-  const uint32_t n = *l.count;
-  const uint32_t total = n ? l.prefix[n - 1] : 0u;
+  const uint32_t n = *glb(l.count);
+  const uint32_t total = n ? glb(l.prefix)[n - 1] : 0u;
   const uint32_t nt = gridDim.x * kLbBlock;
   for (uint32_t base = blockIdx.x * kLbBlock; base < total; base += nt) {
     if (threadIdx.x == 0) {
@@ -132,8 +133,8 @@ __global__ __launch_bounds__(kLbBlock) void lb_relax_kernel(KArgs a, LbLists l) 
     uint32_t e = 0, du = 0;
     if (act) {
       const uint32_t k = lb_find(l.prefix, first, last, w);
-      const uint32_t u = l.src[k];
-      e = a.row_start[u] + (w - (l.prefix[k] - l.deg[k]));
+      const uint32_t u = glb(l.src)[k];
+      e = glb(a.row_start)[u] + (w - (glb(l.prefix)[k] - glb(l.deg)[k]));
       du = ld_coh(a.dist + u);
     }
     relax_edge<A, kLbBlock>(a, s, e, du, act, sh.ps, true);
@@ -159,6 +160,7 @@ __global__ __launch_bounds__(BS) void lb_sweep_kernel(KArgs a) {
     sched_load(ss, &c->s);
     sh.ps.local = 0;
     sh.ps.far_min = kInf;
+    sh.ps.near_lds = sh.ps.in_lds = 0;
     sh.edges = 0;
   }
   __syncthreads();

@@ -65,10 +65,48 @@ struct KArgs {
 // serialises at the memory-side atomic unit (~12 ns each on gfx950).
 constexpr uint32_t kStage = 256;
 
+// The near pile of block-local rounds, kept in LDS: a table of (vertex,
+// smallest near candidate) with the list of its used slots in insertion
+// order, one table per round parity (filled in round r, consumed in r + 1).
+// The table replaces the global near stamps and queue while one block runs
+// rounds on its own: a round then costs the row, edge and atomicMin round
+// trips of its relaxations and no queue or stamp traffic.
+constexpr uint32_t kLocalHash = 1024;  // table slots (power of two)
+constexpr uint32_t kLocalCap = 512;    // listed entries; more set the overflow flag
+constexpr uint32_t kLocalProbe = 32;   // linear-probe bound of an insertion
+struct LocalNear {
+  uint32_t key[2][kLocalHash];   // vertex + 1, 0 = empty
+  uint32_t dist[2][kLocalHash];  // smallest candidate inserted (INF when empty)
+  uint16_t list[2][kLocalCap];   // used slots, insertion order
+  uint32_t n[2];                 // slots claimed (may exceed kLocalCap)
+  uint32_t overflow;             // an insertion found no slot or no list entry this round
+};
+
+// Inserts v with candidate nd into table p; false when the probe path is full
+// (the caller then pushes v to the global near pile).
+__device__ __forceinline__ bool local_insert(LocalNear &ln, uint32_t p, uint32_t v, uint32_t nd) {
+  uint32_t h = (v * 2654435761u) >> 22;  // top 10 bits: kLocalHash = 1024
+  for (uint32_t k = 0; k < kLocalProbe; ++k, h = (h + 1) & (kLocalHash - 1)) {
+    const uint32_t key = atomicCAS(&ln.key[p][h], 0u, v + 1);
+    if (key == 0) {
+      const uint32_t i = atomicAdd(&ln.n[p], 1u);
+      if (i < kLocalCap) ln.list[p][i] = (uint16_t)h;
+      else ln.overflow = 1;
+    }
+    if (key == 0 || key == v + 1) {
+      atomicMin(&ln.dist[p][h], nd);
+      return true;
+    }
+  }
+  ln.overflow = 1;
+  return false;
+}
+
 template <int BS>
 struct PushStage {
   uint32_t item[BS / kWave][2][kStage];  // [wave][near, far]
   uint32_t n[BS / kWave][2];
+  uint32_t cand[BS / kWave][kStage];     // block-local rounds: the staged near items' candidates
   // Bucket far mode: smallest distance this block sent to the far pile this
   // round (pushed candidates and deferred items), folded into the control
   // block's far_min once per round.
@@ -77,7 +115,52 @@ struct PushStage {
   // tails are kept here and stored once at the end of the round.
   uint32_t local;
   uint32_t tail[2];
+  // Block-local rounds: near_lds = near pushes go to table near_par of ln;
+  // in_lds = the round's in-list is table in_par's list.
+  uint32_t near_lds, near_par, in_lds, in_par;
+  uint32_t heavy_seen;  // a heavy vertex was queued as chunks this round
+  LocalNear ln;
 };
+
+// Every block clears its tables once at kernel start (only the block that
+// runs local rounds uses them).
+template <int BS>
+__device__ __forceinline__ void local_init(PushStage<BS> &ps) {
+  for (uint32_t i = threadIdx.x; i < 2 * kLocalHash; i += BS) {
+    (&ps.ln.key[0][0])[i] = 0u;
+    (&ps.ln.dist[0][0])[i] = kInf;
+  }
+  if (threadIdx.x == 0) {
+    ps.ln.n[0] = ps.ln.n[1] = 0;
+    ps.ln.overflow = 0;
+    ps.near_lds = ps.in_lds = 0;
+    ps.heavy_seen = 0;
+  }
+}
+
+// A block-local round whose staged near items outgrow the stage: they go to
+// the global near pile, deduplicated by the near stamps they skipped, and the
+// round ends the block-local run.  Out of line: it is rare, and inlined into
+// every relaxation path its loop cost registers there.
+template <int BS>
+__device__ __noinline__ void stage_restamp(const KArgs &a, const Sched &s, PushStage<BS> &ps, uint32_t w, uint32_t n) {
+  const uint32_t stamp = s.round + 1;
+  if (lane_id() == 0) ps.ln.overflow = 1;
+  for (uint32_t i0 = 0; i0 < n; i0 += kWave) {
+    const uint32_t v = i0 + lane_id() < n ? ps.item[w][0][i0 + lane_id()] : 0u;
+    const bool push = i0 + lane_id() < n && atomic_exch(a.near_stamp + v, stamp) != stamp;
+    const uint64_t m = __ballot(push);
+    uint32_t base = 0;
+    if (lane_id() == 0 && m) base = atomicAdd(&ps.tail[0], (uint32_t)__builtin_popcountll(m));
+    base = readlane(base, 0);
+    if (push) {
+      const uint32_t pos = base + lanes_below(m);
+      if (pos < a.qcap) st_coh(a.q[s.near_q] + pos, v);
+      else st_coh(&a.ctrl->abort, kAbortCapacity);
+    }
+  }
+  ps.n[w][0] = 0;
+}
 
 // Writes this wavefront's staged items of pile `which` (0 near, 1 far) to the queue.
 template <int BS>
@@ -85,6 +168,10 @@ __device__ __forceinline__ void stage_flush(const KArgs &a, const Sched &s, Push
   const uint32_t w = threadIdx.x / kWave;
   const uint32_t n = __builtin_amdgcn_readfirstlane(ps.n[w][which]);
   if (n == 0) return;
+  if (which == 0 && ps.near_lds) {
+    stage_restamp<BS>(a, s, ps, w, n);
+    return;
+  }
   const uint32_t qi = which ? s.far_q : s.near_q;
   uint32_t base = 0;
   if (lane_id() == 0) base = ps.local ? atomicAdd(&ps.tail[which], n) : atomic_add(&a.ctrl->qcount[qi], n);
@@ -95,6 +182,37 @@ __device__ __forceinline__ void stage_flush(const KArgs &a, const Sched &s, Push
     else st_coh(&a.ctrl->abort, kAbortCapacity);
   }
   ps.n[w][which] = 0;
+}
+
+// End of a block-local round: the wavefront's staged near items go into the
+// LDS table, which keeps each vertex once with its smallest candidate; an item
+// whose probe path is full goes to the global near pile instead, deduplicated
+// by the near stamps as in a grid round.
+template <int BS>
+__device__ __forceinline__ void local_absorb(const KArgs &a, const Sched &s, PushStage<BS> &ps) {
+  const uint32_t w = threadIdx.x / kWave;
+  const uint32_t n = __builtin_amdgcn_readfirstlane(ps.n[w][0]);
+  const uint32_t stamp = s.round + 1;
+  for (uint32_t i0 = 0; i0 < n; i0 += kWave) {
+    const uint32_t i = i0 + lane_id();
+    bool spill = false;
+    uint32_t v = 0;
+    if (i < n) {
+      v = ps.item[w][0][i];
+      spill = !local_insert(ps.ln, ps.near_par, v, ps.cand[w][i]) && atomic_exch(a.near_stamp + v, stamp) != stamp;
+    }
+    const uint64_t m = __ballot(spill);
+    if (m == 0) continue;
+    uint32_t base = 0;
+    if (lane_id() == 0) base = atomicAdd(&ps.tail[0], (uint32_t)__builtin_popcountll(m));
+    base = readlane(base, 0);
+    if (spill) {
+      const uint32_t pos = base + lanes_below(m);
+      if (pos < a.qcap) st_coh(a.q[s.near_q] + pos, v);
+      else st_coh(&a.ctrl->abort, kAbortCapacity);
+    }
+  }
+  ps.n[w][0] = 0;
 }
 
 // Pushes `item` from every lane with `pred` onto pile `which`.
@@ -112,6 +230,27 @@ __device__ __forceinline__ void stage_push(const KArgs &a, const Sched &s, PushS
   }
   if (pred) ps.item[w][which][n + lanes_below(m)] = item;
   ps.n[w][which] = n + c;
+}
+
+// A near push of a block-local round: the item and its candidate, staged
+// for the LDS table.
+template <int BS>
+__device__ __forceinline__ void stage_push_local(const KArgs &a, const Sched &s, PushStage<BS> &ps, bool pred,
+                                                 uint32_t item, uint32_t cand) {
+  const uint64_t m = __ballot(pred);
+  if (m == 0) return;
+  const uint32_t w = threadIdx.x / kWave;
+  const uint32_t c = (uint32_t)__builtin_popcountll(m);
+  uint32_t n = __builtin_amdgcn_readfirstlane(ps.n[w][0]);
+  if (n + c > kStage) {
+    stage_flush<BS>(a, s, ps, 0);
+    n = 0;
+  }
+  if (pred) {
+    ps.item[w][0][n + lanes_below(m)] = item;
+    ps.cand[w][n + lanes_below(m)] = cand;
+  }
+  ps.n[w][0] = n + c;
 }
 
 // End of a round: every wavefront writes out what it staged.
@@ -160,17 +299,21 @@ struct SsspRelax {
     const bool near = (int64_t)nd <= s.curdelta();
     uint32_t *stamps = near ? a.near_stamp : a.far_stamp;
     const uint32_t stamp = near ? s.round + 1 : s.phase + 1;
+    // block-local rounds keep near pushes in the LDS table instead
+    const bool lds = ps.near_lds && near;
     uint32_t seen = stamp;
     if (active) {
-      const uint32_t dv = precheck ? a.dist[v] : kInf;
+      const uint32_t dv = precheck ? glb(a.dist)[v] : kInf;
       // the stamp is loaded beside d[v]: one round trip fewer when the edge
       // improves (C1 12.3 -> 11.9 ms, C3 12.0 -> 11.9 ms, RMAT unchanged)
-      seen = stamps[v];
+      seen = lds ? 0u : glb(stamps)[v];
       if (!precheck || nd < dv) imp = nd < atomic_min(a.dist + v, nd);
     }
     // Push once per round (near) or phase (far): the exchange decides between
     // racing improvers, the cached pre-check skips it when already stamped.
-    const bool push = imp && seen != stamp && atomic_exch(stamps + v, stamp) != stamp;
+    // A block-local round stages its near pushes for the LDS table instead.
+    const bool push = imp && !lds && seen != stamp && atomic_exch(stamps + v, stamp) != stamp;
+    if (ps.near_lds) stage_push_local<BS>(a, s, ps, imp && lds, v, nd);
     const bool to_near = push && near, to_far = push && !near;
     if (s.aux[1] == LSG_FAR_BUCKET) note_far_min<BS>(ps, imp && !near, nd);
     stage_push<BS>(a, s, ps, 0, to_near, v);
@@ -189,12 +332,16 @@ struct BfsVisit {
                                               PushStage<BS> &ps, bool precheck) {
     bool imp = false;
     if (active) {
-      const uint32_t dv = precheck ? a.dist[v] : kInf;
+      const uint32_t dv = precheck ? glb(a.dist)[v] : kInf;
       if (!precheck || nd < dv) imp = nd < atomic_min(a.dist + v, nd);
+    }
+    if (ps.near_lds) {
+      stage_push_local<BS>(a, s, ps, imp, v, nd);
+      return;
     }
     if (imp) {
       uint32_t stamp = s.round + 1;
-      imp = a.near_stamp[v] != stamp && atomic_exch(a.near_stamp + v, stamp) != stamp;
+      imp = glb(a.near_stamp)[v] != stamp && atomic_exch(a.near_stamp + v, stamp) != stamp;
     }
     stage_push<BS>(a, s, ps, 0, imp, v);
   }
@@ -205,8 +352,8 @@ __device__ __forceinline__ void relax_edge(const KArgs &a, const Sched &s, uint3
                                            PushStage<BS> &ps, bool precheck) {
   uint32_t v = 0, nd = 0;
   if (active) {
-    v = a.edge_dst[e];
-    nd = A == Algo::SSSP ? du + a.edge_data[e] : du + 1u;
+    v = glb(a.edge_dst)[e];
+    nd = A == Algo::SSSP ? du + glb(a.edge_data)[e] : du + 1u;
   }
   if (A == Algo::SSSP) SsspRelax::edge<BS>(a, s, v, nd, active, ps, precheck);
   else BfsVisit::edge<BS>(a, s, v, nd, active, ps, precheck);
@@ -266,14 +413,24 @@ __device__ void process_tile(const KArgs &a, const Sched &s, uint32_t t, uint32_
                              bool precheck = true) {
   uint32_t i = t * ti + threadIdx.x;
   bool valid = threadIdx.x < ti && i < s.in_count;
-  uint32_t u = valid ? ld_coh(a.q[s.in_q] + i) : 0u;
-  uint32_t du = 0, rs = 0, re = 0, deg = 0;
+  uint32_t u = 0, du = 0, rs = 0, re = 0, deg = 0;
+  if (sh.ps.in_lds) {
+    // a block-local round's in-list: the LDS table's entries, with their
+    // exact distances (only this block has lowered them since)
+    if (valid) {
+      const uint32_t h = sh.ps.ln.list[sh.ps.in_par][i];
+      u = sh.ps.ln.key[sh.ps.in_par][h] - 1;
+      du = sh.ps.ln.dist[sh.ps.in_par][h];
+    }
+  } else {
+    u = valid ? ld_coh(a.q[s.in_q] + i) : 0u;
+    if (valid && u < a.nnodes) du = ld_coh(a.dist + u);
+  }
   valid = valid && u < a.nnodes;
   if (valid) {
     // distance and row bounds in one round trip
-    du = ld_coh(a.dist + u);
-    rs = a.row_start[u];
-    re = a.row_start[u + 1];
+    rs = glb(a.row_start)[u];
+    re = glb(a.row_start)[u + 1];
     if (du == kInf) valid = false;  // INF nodes are never expanded (apps/sssp/kernel.cu:202)
   }
   if (A == Algo::SSSP && s.aux[0] && s.aux[1] != LSG_FAR_REFERENCE) {
@@ -283,7 +440,7 @@ __device__ void process_tile(const KArgs &a, const Sched &s, uint32_t t, uint32_
     if (s.aux[1] == LSG_FAR_BUCKET) note_far_min<BS>(sh.ps, defer, du);
     if (defer) {
       uint32_t stamp = s.phase + 1;
-      defer = a.far_stamp[u] != stamp && atomic_exch(a.far_stamp + u, stamp) != stamp;
+      defer = glb(a.far_stamp)[u] != stamp && atomic_exch(a.far_stamp + u, stamp) != stamp;
       valid = false;
     }
     stage_push<BS>(a, s, sh.ps, 1, defer, u);
@@ -297,6 +454,7 @@ __device__ void process_tile(const KArgs &a, const Sched &s, uint32_t t, uint32_
   bool heavy = deg >= a.heavy_deg;
   uint32_t nch = heavy ? (deg + a.heavy_deg - 1) / a.heavy_deg : 0u;
   if (__ballot(heavy)) {
+    sh.ps.heavy_seen = 1;
     uint32_t hq = 1u - s.heavy_in;
     uint32_t base = wave_reserve(nch, &a.ctrl->hcount[hq]);
     uint64_t *dst = reinterpret_cast<uint64_t *>(a.hq[hq]);
@@ -373,7 +531,7 @@ __device__ void process_heavy(const KArgs &a, const Sched &s, uint32_t c, TileSh
   uint32_t u = (uint32_t)ent, k = (uint32_t)(ent >> 32);
   if (u >= a.nnodes) return;
   uint32_t du = ld_coh(a.dist + u);
-  uint32_t rs = a.row_start[u], re = a.row_start[u + 1];
+  uint32_t rs = glb(a.row_start)[u], re = glb(a.row_start)[u + 1];
   if ((uint64_t)k * a.heavy_deg >= re - rs) return;
   uint32_t lo = rs + k * a.heavy_deg;
   uint32_t hi = min(re, lo + a.heavy_deg);
@@ -531,33 +689,136 @@ __device__ __forceinline__ uint32_t tile_items(uint32_t in_count, uint32_t nbloc
 constexpr uint32_t kLocalTiles = 1;
 
 // `ls` and `labort`: the schedule and abort flag the leader step just left in LDS.
+//
+// While a local round's near output fits one tile, has no heavy vertex and
+// found room in the LDS table, the next round takes its in-list from the
+// table and no word of the schedule, queues or statistics goes through
+// global memory: the round's bookkeeping (round number, statistics, trace
+// record) stays in LDS.  Otherwise, and at the end of a phase, the table's
+// vertices join the global near pile (with the near stamps, so that vertices
+// a full probe path sent there are not queued twice) and the leader step runs
+// as after a grid round.
 template <Algo A, int BS>
 __device__ void local_rounds(const KArgs &a, TileShared<BS> &sh, Sched &ls, uint32_t &labort, uint32_t &lfar) {
-  __shared__ uint32_t go;
+  __shared__ uint32_t go, stay;
+  __shared__ uint64_t acc_items, acc_rounds;
   Ctrl *c = a.ctrl;
+  PushStage<BS> &ps = sh.ps;
   for (;;) {
     if (threadIdx.x == 0) {
       go = !ls.done && ls.heavy_count == 0 && ls.in_count <= kLocalTiles * BS && labort == kAbortNone;
       sh.edges = 0;
-      sh.ps.local = go;
-      sh.ps.tail[0] = 0;  // the near pile starts empty
-      sh.ps.tail[1] = lfar;
+      ps.local = go;
+      ps.tail[0] = 0;  // the near pile starts empty
+      ps.tail[1] = lfar;
+      ps.in_lds = 0;
+      acc_items = acc_rounds = 0;
     }
     __syncthreads();
     if (!go) return;
-    const Sched s = ls;
-    for (uint32_t t = 0; t * BS < s.in_count; ++t) {
-      process_tile<A, BS>(a, s, t, BS, sh, false);
+    for (;;) {
+      const Sched s = ls;
+      const uint32_t par = (s.round + 1) & 1;
+      if (threadIdx.x == 0) {
+        ps.near_lds = 1;
+        ps.near_par = par;
+        ps.heavy_seen = 0;
+        ps.ln.overflow = 0;
+      }
       __syncthreads();
+      for (uint32_t t = 0; t * BS < s.in_count; ++t) {
+        process_tile<A, BS>(a, s, t, BS, sh, false);
+        __syncthreads();
+      }
+      local_absorb<BS>(a, s, ps);  // each wavefront's staged near items into the table
+      if (ps.in_lds) {
+        // the consumed table is empty again for round r + 2
+        const uint32_t ip = ps.in_par;
+        for (uint32_t i = threadIdx.x; i < s.in_count; i += BS) {
+          const uint32_t h = ps.ln.list[ip][i];
+          ps.ln.key[ip][h] = 0u;
+          ps.ln.dist[ip][h] = kInf;
+        }
+        if (threadIdx.x == 0) ps.ln.n[ip] = 0;
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        const uint32_t n = ps.ln.n[par];
+        // (a full probe path or list sets overflow; only then did a near
+        // push go to the global pile)
+        const bool more = n > 0 && n <= BS && !ps.ln.overflow && !ps.heavy_seen;
+        stay = more;
+        if (more) {
+          // the leader step's bookkeeping of a round that refills the near
+          // pile, kept in LDS (apps/sssp/kernel.cu:370-372: retry2)
+          acc_items += s.in_count;
+          acc_rounds += 1;
+          if (a.trace && s.round - 1 < a.trace_cap) {
+            lsg_round_trace &r = a.trace[s.round - 1];
+            r.round = s.round;
+            r.phase = s.phase;
+            r.in_items = s.in_count;
+            r.heavy_chunks = 0;
+            r.near_out = n;
+            r.far_out = ps.tail[1];
+            r.heavy_out = 0;
+            r.curdelta_lo = s.curdelta_lo;
+            r.curdelta_hi = s.curdelta_hi;
+            r.delta = s.delta;
+            r.t_end = wall_clock64();
+          }
+          ls.round += 1;
+          ls.in_count = n;
+          ls.aux[0] = 0;
+          ps.in_lds = 1;
+          ps.in_par = par;
+          st_coh(&c->progress, s.round);
+        }
+      }
+      __syncthreads();
+      if (!stay) break;
     }
-    stage_drain<BS>(a, s, sh.ps);
+    // The table's vertices join the global near pile; its slots are cleared.
+    const Sched s = ls;
+    const uint32_t par = (s.round + 1) & 1;
+    const uint32_t stamp = s.round + 1;
+    if (threadIdx.x == 0) ps.near_lds = 0;
+    __syncthreads();
+    for (uint32_t h0 = 0; h0 < kLocalHash; h0 += BS) {
+      const uint32_t h = h0 + threadIdx.x;
+      const uint32_t key = ps.ln.key[par][h];
+      bool push = key != 0;
+      if (push) {
+        ps.ln.key[par][h] = 0u;
+        ps.ln.dist[par][h] = kInf;
+        push = atomic_exch(a.near_stamp + (key - 1), stamp) != stamp;
+      }
+      stage_push<BS>(a, s, ps, 0, push, key - 1);
+    }
+    if (threadIdx.x == 0) ps.ln.n[par] = 0;
+    if (ps.in_lds) {
+      // clear the other table too, in case its consumed list was not complete
+      for (uint32_t h = threadIdx.x; h < kLocalHash; h += BS) {
+        ps.ln.key[ps.in_par][h] = 0u;
+        ps.ln.dist[ps.in_par][h] = kInf;
+      }
+    }
+    stage_drain<BS>(a, s, ps);
     __syncthreads();  // every wavefront has taken its places from the LDS tails
     if (threadIdx.x == 0) {
-      st_coh(&c->qcount[s.near_q], sh.ps.tail[0]);
-      st_coh(&c->qcount[s.far_q], sh.ps.tail[1]);
-      flush_far_min<BS>(a, sh.ps);
+      st_coh(&c->qcount[s.near_q], ps.tail[0]);
+      st_coh(&c->qcount[s.far_q], ps.tail[1]);
+      flush_far_min<BS>(a, ps);
       if ((a.sched || a.feat) && sh.edges)
         __hip_atomic_fetch_add(&c->stat_edges, (uint64_t)sh.edges, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (acc_rounds) {
+        // rounds run from the LDS table, before the leader step adds this one
+        st_coh64(&c->stat_items, ld_coh64(&c->stat_items) + acc_items);
+        st_coh64(&c->stat_rounds, ld_coh64(&c->stat_rounds) + acc_rounds);
+      }
+      sched_store(&c->s, ls);
+      ps.in_lds = 0;
+      ps.ln.n[0] = ps.ln.n[1] = 0;
     }
     // this block's pushes and counts are complete before the leader step reads them
     order_release();
@@ -583,6 +844,7 @@ __global__ __launch_bounds__(BS) void persistent_kernel(KArgs a) {
     sh.ps.local = 0;
     sh.ps.far_min = kInf;
   }
+  local_init<BS>(sh.ps);
   for (;;) {
     if (threadIdx.x == 0) {
       Sched s;
@@ -630,9 +892,9 @@ __global__ __launch_bounds__(BS) void persistent_kernel(KArgs a) {
 static __global__ void init_kernel(KArgs a, uint32_t delta) {
   uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
   for (uint32_t v = tid; v < a.nnodes; v += nt) {
-    a.dist[v] = kInf;
-    a.near_stamp[v] = 0u;
-    a.far_stamp[v] = 0u;
+    glb(a.dist)[v] = kInf;
+    glb(a.near_stamp)[v] = 0u;
+    glb(a.far_stamp)[v] = 0u;
   }
   ctrl_reset_barrier(a.ctrl, tid, nt);
   if (tid == 0) {

@@ -92,11 +92,11 @@ def test_rmat20_config2_sssp(rmat20):
     src = g.max_degree_vertex()
     ref = oracle.dijkstra(g.row_start, g.edge_dst, g.edge_data, src)
     dev = g.copy_to_gpu()
-    for mode in (dict(), dict(expand_mode=1), dict(block_size=512, heavy_deg=1024), dict(far_mode=1),
-                 dict(far_mode=2), dict(far_mode=2, expand_mode=0, delta=7)):
-        say("rmat20 sssp", mode)
-        d = run_sssp(g, dev, src, 32, **mode)
-        assert np.array_equal(d, ref), mode
+    for delta, mode in ((32, dict()), (32, dict(expand_mode=1)), (32, dict(block_size=512, heavy_deg=1024)),
+                        (32, dict(far_mode=1)), (32, dict(far_mode=2)), (7, dict(far_mode=2, expand_mode=0))):
+        say("rmat20 sssp", delta, mode)
+        d = run_sssp(g, dev, src, delta, **mode)
+        assert np.array_equal(d, ref), (delta, mode)
     # another source, reference default delta
     src2 = int(np.flatnonzero(g.out_degree() > 0)[12345])
     ref2 = oracle.dijkstra(g.row_start, g.edge_dst, g.edge_data, src2)
