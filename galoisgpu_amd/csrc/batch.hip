@@ -526,7 +526,7 @@ __device__ __forceinline__ void process_chunk(const MArgs &a, const Sched &s, La
 // plus, at each phase boundary, the learned scheduler's choice of the next
 // phase's delta width and far-pile mode.  Schedule words: aux[0] = the
 // in-queue is the far pile, aux[1] = far mode of this phase (LSG_FAR_*).
-__device__ void multi_leader(const MArgs &a, MultiCounters *tot) {
+__device__ __forceinline__ void multi_leader(const MArgs &a, MultiCounters *tot) {
   __shared__ Sched s;
   __shared__ SchedFeatures f;
   __shared__ MlpScratch sc;

@@ -317,7 +317,7 @@ __device__ __forceinline__ void pdo_bottom_up(const PdoArgs &a, const Sched &s, 
 // estimate, and the direction of level L + 1 (the rule of the host loop it
 // replaces: bottom-up once the frontier's edges exceed 1/alpha of the
 // unexplored edges, top-down again below 1/beta of the vertices).
-__device__ void pdo_leader(const PdoArgs &a) {
+__device__ __forceinline__ void pdo_leader(const PdoArgs &a) {
   __shared__ uint32_t tot[2];
   if (threadIdx.x < 2) tot[threadIdx.x] = 0;
   __syncthreads();

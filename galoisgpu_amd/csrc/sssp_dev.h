@@ -106,7 +106,11 @@ template <int BS>
 struct PushStage {
   uint32_t item[BS / kWave][2][kStage];  // [wave][near, far]
   uint32_t n[BS / kWave][2];
-  uint32_t cand[BS / kWave][kStage];     // block-local rounds: the staged near items' candidates
+  // Block-local rounds: near items (without stamps) and their candidates,
+  // staged for the LDS table.
+  uint32_t litem[BS / kWave][kStage];
+  uint32_t lcand[BS / kWave][kStage];
+  uint32_t ln_n[BS / kWave];
   // Bucket far mode: smallest distance this block sent to the far pile this
   // round (pushed candidates and deferred items), folded into the control
   // block's far_min once per round.
@@ -115,9 +119,9 @@ struct PushStage {
   // tails are kept here and stored once at the end of the round.
   uint32_t local;
   uint32_t tail[2];
-  // Block-local rounds: near_lds = near pushes go to table near_par of ln;
-  // in_lds = the round's in-list is table in_par's list.
-  uint32_t near_lds, near_par, in_lds, in_par;
+  // Block-local rounds: near pushes go to table near_par of ln; in_lds = the
+  // round's in-list is table in_par's list.
+  uint32_t near_par, in_lds, in_par;
   uint32_t heavy_seen;  // a heavy vertex was queued as chunks this round
   LocalNear ln;
 };
@@ -130,36 +134,13 @@ __device__ __forceinline__ void local_init(PushStage<BS> &ps) {
     (&ps.ln.key[0][0])[i] = 0u;
     (&ps.ln.dist[0][0])[i] = kInf;
   }
+  if (threadIdx.x < BS / kWave) ps.ln_n[threadIdx.x] = 0;
   if (threadIdx.x == 0) {
     ps.ln.n[0] = ps.ln.n[1] = 0;
     ps.ln.overflow = 0;
-    ps.near_lds = ps.in_lds = 0;
+    ps.in_lds = 0;
     ps.heavy_seen = 0;
   }
-}
-
-// A block-local round whose staged near items outgrow the stage: they go to
-// the global near pile, deduplicated by the near stamps they skipped, and the
-// round ends the block-local run.  Out of line: it is rare, and inlined into
-// every relaxation path its loop cost registers there.
-template <int BS>
-__device__ __noinline__ void stage_restamp(const KArgs &a, const Sched &s, PushStage<BS> &ps, uint32_t w, uint32_t n) {
-  const uint32_t stamp = s.round + 1;
-  if (lane_id() == 0) ps.ln.overflow = 1;
-  for (uint32_t i0 = 0; i0 < n; i0 += kWave) {
-    const uint32_t v = i0 + lane_id() < n ? ps.item[w][0][i0 + lane_id()] : 0u;
-    const bool push = i0 + lane_id() < n && atomic_exch(a.near_stamp + v, stamp) != stamp;
-    const uint64_t m = __ballot(push);
-    uint32_t base = 0;
-    if (lane_id() == 0 && m) base = atomicAdd(&ps.tail[0], (uint32_t)__builtin_popcountll(m));
-    base = readlane(base, 0);
-    if (push) {
-      const uint32_t pos = base + lanes_below(m);
-      if (pos < a.qcap) st_coh(a.q[s.near_q] + pos, v);
-      else st_coh(&a.ctrl->abort, kAbortCapacity);
-    }
-  }
-  ps.n[w][0] = 0;
 }
 
 // Writes this wavefront's staged items of pile `which` (0 near, 1 far) to the queue.
@@ -168,10 +149,6 @@ __device__ __forceinline__ void stage_flush(const KArgs &a, const Sched &s, Push
   const uint32_t w = threadIdx.x / kWave;
   const uint32_t n = __builtin_amdgcn_readfirstlane(ps.n[w][which]);
   if (n == 0) return;
-  if (which == 0 && ps.near_lds) {
-    stage_restamp<BS>(a, s, ps, w, n);
-    return;
-  }
   const uint32_t qi = which ? s.far_q : s.near_q;
   uint32_t base = 0;
   if (lane_id() == 0) base = ps.local ? atomicAdd(&ps.tail[which], n) : atomic_add(&a.ctrl->qcount[qi], n);
@@ -191,28 +168,19 @@ __device__ __forceinline__ void stage_flush(const KArgs &a, const Sched &s, Push
 template <int BS>
 __device__ __forceinline__ void local_absorb(const KArgs &a, const Sched &s, PushStage<BS> &ps) {
   const uint32_t w = threadIdx.x / kWave;
-  const uint32_t n = __builtin_amdgcn_readfirstlane(ps.n[w][0]);
+  const uint32_t n = __builtin_amdgcn_readfirstlane(ps.ln_n[w]);
   const uint32_t stamp = s.round + 1;
   for (uint32_t i0 = 0; i0 < n; i0 += kWave) {
     const uint32_t i = i0 + lane_id();
     bool spill = false;
     uint32_t v = 0;
     if (i < n) {
-      v = ps.item[w][0][i];
-      spill = !local_insert(ps.ln, ps.near_par, v, ps.cand[w][i]) && atomic_exch(a.near_stamp + v, stamp) != stamp;
+      v = ps.litem[w][i];
+      spill = !local_insert(ps.ln, ps.near_par, v, ps.lcand[w][i]) && atomic_exch(a.near_stamp + v, stamp) != stamp;
     }
-    const uint64_t m = __ballot(spill);
-    if (m == 0) continue;
-    uint32_t base = 0;
-    if (lane_id() == 0) base = atomicAdd(&ps.tail[0], (uint32_t)__builtin_popcountll(m));
-    base = readlane(base, 0);
-    if (spill) {
-      const uint32_t pos = base + lanes_below(m);
-      if (pos < a.qcap) st_coh(a.q[s.near_q] + pos, v);
-      else st_coh(&a.ctrl->abort, kAbortCapacity);
-    }
+    stage_push<BS>(a, s, ps, 0, spill, v);
   }
-  ps.n[w][0] = 0;
+  ps.ln_n[w] = 0;
 }
 
 // Pushes `item` from every lane with `pred` onto pile `which`.
@@ -233,7 +201,9 @@ __device__ __forceinline__ void stage_push(const KArgs &a, const Sched &s, PushS
 }
 
 // A near push of a block-local round: the item and its candidate, staged
-// for the LDS table.
+// for the LDS table.  When the wavefront's local stage is full, the items
+// take the grid path instead (near stamps, the global near pile) and the
+// block-local run ends after this round.
 template <int BS>
 __device__ __forceinline__ void stage_push_local(const KArgs &a, const Sched &s, PushStage<BS> &ps, bool pred,
                                                  uint32_t item, uint32_t cand) {
@@ -241,16 +211,19 @@ __device__ __forceinline__ void stage_push_local(const KArgs &a, const Sched &s,
   if (m == 0) return;
   const uint32_t w = threadIdx.x / kWave;
   const uint32_t c = (uint32_t)__builtin_popcountll(m);
-  uint32_t n = __builtin_amdgcn_readfirstlane(ps.n[w][0]);
-  if (n + c > kStage) {
-    stage_flush<BS>(a, s, ps, 0);
-    n = 0;
+  const uint32_t n = __builtin_amdgcn_readfirstlane(ps.ln_n[w]);
+  if (n + c <= kStage) {
+    if (pred) {
+      ps.litem[w][n + lanes_below(m)] = item;
+      ps.lcand[w][n + lanes_below(m)] = cand;
+    }
+    ps.ln_n[w] = n + c;
+    return;
   }
-  if (pred) {
-    ps.item[w][0][n + lanes_below(m)] = item;
-    ps.cand[w][n + lanes_below(m)] = cand;
-  }
-  ps.n[w][0] = n + c;
+  if (lane_id() == 0) ps.ln.overflow = 1;
+  const uint32_t stamp = s.round + 1;
+  const bool push = pred && atomic_exch(a.near_stamp + item, stamp) != stamp;
+  stage_push<BS>(a, s, ps, 0, push, item);
 }
 
 // End of a round: every wavefront writes out what it staged.
@@ -286,8 +259,9 @@ __device__ __forceinline__ void flush_far_min(const KArgs &a, PushStage<BS> &ps)
 // really lowered d[v] (the reference pushes on the pre-check alone), and the
 // near/far split at curdelta (:134).  Arithmetic is u32: d[u] < 2^31 and
 // w <= INT_MAX never wrap, and a candidate >= INT_MAX never beats INF.
+// L: a block-local round, whose near pushes go to the LDS table.
 struct SsspRelax {
-  template <int BS>
+  template <int BS, bool L>
   __device__ __forceinline__ static void edge(const KArgs &a, const Sched &s, uint32_t v, uint32_t nd, bool active,
                                               PushStage<BS> &ps, bool precheck) {
     // The pre-checks read through the cache: a stale d[v] is only ever larger
@@ -300,7 +274,7 @@ struct SsspRelax {
     uint32_t *stamps = near ? a.near_stamp : a.far_stamp;
     const uint32_t stamp = near ? s.round + 1 : s.phase + 1;
     // block-local rounds keep near pushes in the LDS table instead
-    const bool lds = ps.near_lds && near;
+    const bool lds = L && near;
     uint32_t seen = stamp;
     if (active) {
       const uint32_t dv = precheck ? glb(a.dist)[v] : kInf;
@@ -313,7 +287,7 @@ struct SsspRelax {
     // racing improvers, the cached pre-check skips it when already stamped.
     // A block-local round stages its near pushes for the LDS table instead.
     const bool push = imp && !lds && seen != stamp && atomic_exch(stamps + v, stamp) != stamp;
-    if (ps.near_lds) stage_push_local<BS>(a, s, ps, imp && lds, v, nd);
+    if (L) stage_push_local<BS>(a, s, ps, imp && lds, v, nd);
     const bool to_near = push && near, to_far = push && !near;
     if (s.aux[1] == LSG_FAR_BUCKET) note_far_min<BS>(ps, imp && !near, nd);
     stage_push<BS>(a, s, ps, 0, to_near, v);
@@ -327,7 +301,7 @@ struct SsspRelax {
 // one round after their vertex is popped) still ends at its hop count; every
 // improvement is pushed to the next level's pile, once per round.
 struct BfsVisit {
-  template <int BS>
+  template <int BS, bool L>
   __device__ __forceinline__ static void edge(const KArgs &a, const Sched &s, uint32_t v, uint32_t nd, bool active,
                                               PushStage<BS> &ps, bool precheck) {
     bool imp = false;
@@ -335,7 +309,7 @@ struct BfsVisit {
       const uint32_t dv = precheck ? glb(a.dist)[v] : kInf;
       if (!precheck || nd < dv) imp = nd < atomic_min(a.dist + v, nd);
     }
-    if (ps.near_lds) {
+    if (L) {
       stage_push_local<BS>(a, s, ps, imp, v, nd);
       return;
     }
@@ -347,7 +321,7 @@ struct BfsVisit {
   }
 };
 
-template <Algo A, int BS>
+template <Algo A, int BS, bool L = false>
 __device__ __forceinline__ void relax_edge(const KArgs &a, const Sched &s, uint32_t e, uint32_t du, bool active,
                                            PushStage<BS> &ps, bool precheck) {
   uint32_t v = 0, nd = 0;
@@ -355,8 +329,8 @@ __device__ __forceinline__ void relax_edge(const KArgs &a, const Sched &s, uint3
     v = glb(a.edge_dst)[e];
     nd = A == Algo::SSSP ? du + glb(a.edge_data)[e] : du + 1u;
   }
-  if (A == Algo::SSSP) SsspRelax::edge<BS>(a, s, v, nd, active, ps, precheck);
-  else BfsVisit::edge<BS>(a, s, v, nd, active, ps, precheck);
+  if (A == Algo::SSSP) SsspRelax::edge<BS, L>(a, s, v, nd, active, ps, precheck);
+  else BfsVisit::edge<BS, L>(a, s, v, nd, active, ps, precheck);
 }
 
 // ---- block-level helpers ---------------------------------------------------
@@ -408,13 +382,13 @@ struct TileShared {
 // each popped vertex (the reference's pop + filter, apps/sssp/kernel.cu:201-210),
 // queues heavy vertices as chunks for the next round, and expands the rest by
 // the selected mode.
-template <Algo A, int BS>
-__device__ void process_tile(const KArgs &a, const Sched &s, uint32_t t, uint32_t ti, TileShared<BS> &sh,
+template <Algo A, int BS, bool L = false>
+__device__ __forceinline__ void process_tile(const KArgs &a, const Sched &s, uint32_t t, uint32_t ti, TileShared<BS> &sh,
                              bool precheck = true) {
   uint32_t i = t * ti + threadIdx.x;
   bool valid = threadIdx.x < ti && i < s.in_count;
   uint32_t u = 0, du = 0, rs = 0, re = 0, deg = 0;
-  if (sh.ps.in_lds) {
+  if (L && sh.ps.in_lds) {
     // a block-local round's in-list: the LDS table's entries, with their
     // exact distances (only this block has lowered them since)
     if (valid) {
@@ -498,7 +472,7 @@ __device__ void process_tile(const KArgs &a, const Sched &s, uint32_t t, uint32_
         ee = sh.start[lo] + (e - sh.offs[lo]);
         d0 = sh.du[lo];
       }
-      relax_edge<A, BS>(a, s, ee, d0, act, sh.ps, precheck);
+      relax_edge<A, BS, L>(a, s, ee, d0, act, sh.ps, precheck);
     }
     __syncthreads();
   } else {
@@ -509,7 +483,7 @@ __device__ void process_tile(const KArgs &a, const Sched &s, uint32_t t, uint32_
     uint32_t mx = my;
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, off));
-    for (uint32_t j = 0; j < mx; ++j) relax_edge<A, BS>(a, s, rs + j, du, j < my, sh.ps, precheck);
+    for (uint32_t j = 0; j < mx; ++j) relax_edge<A, BS, L>(a, s, rs + j, du, j < my, sh.ps, precheck);
     uint64_t med = __ballot(!small && deg > 0);
     while (med) {
       uint32_t l = (uint32_t)__builtin_ctzll(med);
@@ -517,7 +491,7 @@ __device__ void process_tile(const KArgs &a, const Sched &s, uint32_t t, uint32_
       uint32_t ws = readlane(rs, l), wd = readlane(deg, l), wu = readlane(du, l);
       for (uint32_t j = 0; j < wd; j += kWave) {
         uint32_t jj = j + lane_id();
-        relax_edge<A, BS>(a, s, ws + jj, wu, jj < wd, sh.ps, precheck);
+        relax_edge<A, BS, L>(a, s, ws + jj, wu, jj < wd, sh.ps, precheck);
       }
     }
   }
@@ -526,7 +500,7 @@ __device__ void process_tile(const KArgs &a, const Sched &s, uint32_t t, uint32_
 // One heavy chunk: heavy_deg consecutive edges of one vertex, spread over
 // the block (replaces the -l edge-balanced TB_LB pass, apps/sssp/kernel.cu:78-149).
 template <Algo A, int BS>
-__device__ void process_heavy(const KArgs &a, const Sched &s, uint32_t c, TileShared<BS> &sh) {
+__device__ __forceinline__ void process_heavy(const KArgs &a, const Sched &s, uint32_t c, TileShared<BS> &sh) {
   uint64_t ent = ld_coh64(reinterpret_cast<const uint64_t *>(a.hq[s.heavy_in]) + c);
   uint32_t u = (uint32_t)ent, k = (uint32_t)(ent >> 32);
   if (u >= a.nnodes) return;
@@ -554,7 +528,7 @@ __device__ void process_heavy(const KArgs &a, const Sched &s, uint32_t c, TileSh
 // counters and the far pile's count for the next round (its near pile starts
 // empty), so rounds that the same block runs next need not load them again.
 template <Algo A>
-__device__ void leader_step(const KArgs &a, Sched &s, uint32_t &abort_seen, uint32_t &far_tail) {
+__device__ __forceinline__ void leader_step(const KArgs &a, Sched &s, uint32_t &abort_seen, uint32_t &far_tail) {
   __shared__ SchedFeatures f;
   __shared__ MlpScratch sc;
   __shared__ uint32_t decide, consumed, far_min;
@@ -699,7 +673,7 @@ constexpr uint32_t kLocalTiles = 1;
 // a full probe path sent there are not queued twice) and the leader step runs
 // as after a grid round.
 template <Algo A, int BS>
-__device__ void local_rounds(const KArgs &a, TileShared<BS> &sh, Sched &ls, uint32_t &labort, uint32_t &lfar) {
+__device__ __forceinline__ void local_rounds(const KArgs &a, TileShared<BS> &sh, Sched &ls, uint32_t &labort, uint32_t &lfar) {
   __shared__ uint32_t go, stay;
   __shared__ uint64_t acc_items, acc_rounds;
   Ctrl *c = a.ctrl;
@@ -720,14 +694,13 @@ __device__ void local_rounds(const KArgs &a, TileShared<BS> &sh, Sched &ls, uint
       const Sched s = ls;
       const uint32_t par = (s.round + 1) & 1;
       if (threadIdx.x == 0) {
-        ps.near_lds = 1;
         ps.near_par = par;
         ps.heavy_seen = 0;
         ps.ln.overflow = 0;
       }
       __syncthreads();
       for (uint32_t t = 0; t * BS < s.in_count; ++t) {
-        process_tile<A, BS>(a, s, t, BS, sh, false);
+        process_tile<A, BS, true>(a, s, t, BS, sh, false);
         __syncthreads();
       }
       local_absorb<BS>(a, s, ps);  // each wavefront's staged near items into the table
@@ -782,8 +755,7 @@ __device__ void local_rounds(const KArgs &a, TileShared<BS> &sh, Sched &ls, uint
     const Sched s = ls;
     const uint32_t par = (s.round + 1) & 1;
     const uint32_t stamp = s.round + 1;
-    if (threadIdx.x == 0) ps.near_lds = 0;
-    __syncthreads();
+
     for (uint32_t h0 = 0; h0 < kLocalHash; h0 += BS) {
       const uint32_t h = h0 + threadIdx.x;
       const uint32_t key = ps.ln.key[par][h];
