@@ -7,16 +7,10 @@
 // (apps/sssp/kernel.cu:129-144,355-391) -- so its distances are identical to a
 // single-source run; what changes is that all sources share every round.
 //
-// Layout in HBM: the distances of v from the S <= 64 sources form row v of a
-// two-plane matrix: plane A holds sources 0..31 in rows of 2^shA words (the
-// smallest power of two >= min(S, 32)), plane B sources 32..S-1 in rows of
-// 2^shB words.  The 64 lanes of a wavefront carry the 64 sources, so relaxing
-// edge u->v for every source is one coalesced row load and masked atomicMin
-// on the same row, and the edge's column and weight are read once for all
-// sources.  Every plane row lies inside one 128-byte line, so a row access
-// costs one line per plane it needs (a plane none of the item's sources is
-// in is not touched); compact rows of S words straddle two lines for any S
-// that is not a power of two (33 sources: 132-byte rows, two lines each).  Per-vertex 64-bit
+// Layout in HBM: D[v][s] (u32, row stride S <= 64): the 64 lanes of a
+// wavefront carry the 64 sources, so relaxing edge u->v for every source is one
+// coalesced row load of D[v] and masked atomicMin on the same row, and the
+// edge's column and weight are read once for all sources.  Per-vertex 64-bit
 // masks say which sources must expand a vertex: nmask[2][V] for the near pile
 // (indexed by round parity: written in round r, consumed in round r+1) and
 // fmask[2][V] for the far pile (phase parity).  A vertex enters a pile's queue
@@ -54,9 +48,7 @@ struct MArgs {
   const uint32_t *row_start;
   const uint32_t *edge_dst;
   const uint32_t *edge_data;
-  uint32_t *DA;         // plane A: [V][2^shA], sources 0..min(S,32)-1
-  uint32_t *DB;         // plane B: [V][2^shB], sources 32..S-1 (S > 32 only)
-  uint32_t shA, shB;
+  uint32_t *D;          // [V][S]
   uint64_t *nmask[2];   // near masks by round parity
   uint64_t *fmask[2];   // far masks by phase parity
   uint32_t *q[3];       // item queues (vertex ids)
@@ -92,34 +84,7 @@ struct LaneSched {
   uint32_t thr32; // threshold of this lane as u32: a candidate nd goes near iff nd <= thr32
   uint32_t *pb;   // this wavefront's LDS push buffers: [0, kPushBuf) near, [kPushBuf, 2 kPushBuf) far
   uint32_t pbn[2];  // items staged per pile (wave-uniform)
-  uint32_t offA, offB;  // word of a plane-A / plane-B row this lane loads when it probes that plane
-  bool homeA;           // this lane's source (if any) is in plane A
 };
-
-// This lane's own cell of row v (its source's distance).
-__device__ __forceinline__ uint32_t *cell(const MArgs &a, const LaneSched &ls, uint32_t v) {
-  return ls.homeA ? a.DA + ((size_t)v << a.shA) + ls.offA : a.DB + ((size_t)v << a.shB) + ls.offB;
-}
-
-// Word of row v this lane loads when the sources with a candidate lie in
-// plane A (needA) and / or plane B (needB): its own cell when its plane is
-// needed, else a word of the needed plane's row, so that the wavefront
-// touches only the lines of the planes it needs.  A lane with a candidate
-// always reads its own cell.
-__device__ __forceinline__ const uint32_t *probe(const MArgs &a, const LaneSched &ls, uint32_t v, bool needA,
-                                                 bool needB) {
-  const bool useA = a.S <= 32 || (ls.homeA ? (needA || !needB) : !needB);
-  return useA ? a.DA + ((size_t)v << a.shA) + ls.offA : a.DB + ((size_t)v << a.shB) + ls.offB;
-}
-
-// Per-lane constants of the two-plane layout.
-__device__ __forceinline__ void lane_cols(const MArgs &a, LaneSched &ls) {
-  const uint32_t lane = lane_id();
-  const uint32_t sa = min(a.S, 32u);
-  ls.homeA = lane < 32 || a.S <= 32;
-  ls.offA = min(lane & 31u, sa - 1);
-  ls.offB = a.S > 32 ? min(lane < 32 ? lane : lane - 32, a.S - 33) : 0u;
-}
 
 // Writes pile `which`'s staged items to its queue (one atomic for all).
 __device__ __forceinline__ void push_flush(const MArgs &a, const Sched &s, LaneSched &ls, int which) {
@@ -185,6 +150,9 @@ __device__ __forceinline__ void relax_batch(const MArgs &a, const Sched &s, Lane
   // Near/far threshold of this lane: the shared curdelta (apps/sssp/kernel.cu:134)
   // or, in a bucket phase, the lane's own source threshold.
   const int64_t thr = ls.bucket ? (int64_t)ls.thr : s.curdelta();
+  // Column this lane loads: its own source, or for lanes >= S (no source, no
+  // candidate) the row's last word, so that every load stays inside row v.
+  const uint32_t col = min(lane, a.S - 1);
   uint64_t my_nb = 0, my_fb = 0;
   for (uint32_t j0 = 0; j0 < n; j0 += kUnroll) {
     uint32_t v[kUnroll], nd[kUnroll], cur[kUnroll], old[kUnroll];
@@ -198,18 +166,15 @@ __device__ __forceinline__ void relax_batch(const MArgs &a, const Sched &s, Lane
       nd[k] = du == kInf || j0 + k >= n ? kInf : du + readlane(ew, j);
     }
     // ... then the group's row loads, unconditional so that none waits on a
-    // branch: every lane loads a word of a row line that is fetched anyway
-    // for the lanes with a candidate.
+    // branch: every lane loads a word of D[v], whose row (S words) is fetched
+    // anyway when any lane needs it.
 #pragma unroll
-    for (int k = 0; k < kUnroll; ++k) {
-      const uint64_t cm = __ballot(nd[k] < kInf);
-      cur[k] = ld_coh(probe(a, ls, v[k], (uint32_t)cm != 0, (cm >> 32) != 0));
-    }
+    for (int k = 0; k < kUnroll; ++k) cur[k] = ld_coh(a.D + (size_t)v[k] * a.S + col);
     // every atomicMin of the group is issued before any result is used
 #pragma unroll
     for (int k = 0; k < kUnroll; ++k) {
       want[k] = nd[k] < cur[k];
-      old[k] = want[k] ? atomic_min(cell(a, ls, v[k]), nd[k]) : kInf;
+      old[k] = want[k] ? atomic_min(a.D + (size_t)v[k] * a.S + lane, nd[k]) : kInf;
     }
 #pragma unroll
     for (int k = 0; k < kUnroll; ++k) {
@@ -292,8 +257,8 @@ struct Window {
 // Issues the row loads of a window of edges (destinations vv, weights ww)
 // leaving an item whose distance for this lane's source is du (INF: no
 // candidate); the first `skip` edges carry no candidate (relaxed already).
-__device__ __forceinline__ void window_load(const MArgs &a, const LaneSched &ls, uint32_t du, const U32xG &vv,
-                                            const U32xG &ww, uint32_t skip, bool needA, bool needB, Window &w) {
+__device__ __forceinline__ void window_load(const MArgs &a, uint32_t du, const U32xG &vv, const U32xG &ww,
+                                            uint32_t skip, uint32_t col, Window &w) {
 #pragma unroll
   for (int k = 0; k < kGroup; ++k) {
     // du + w never wraps (du <= INF, w <= INT_MAX); INF + w and a skipped
@@ -301,13 +266,14 @@ __device__ __forceinline__ void window_load(const MArgs &a, const LaneSched &ls,
     w.nd[k] = du + ((uint32_t)k < skip ? 0x80000000u : ww[k]);
   }
 #pragma unroll
-  for (int k = 0; k < kGroup; ++k) w.val[k] = ld_coh(probe(a, ls, vv[k], needA, needB));
+  for (int k = 0; k < kGroup; ++k) w.val[k] = ld_coh(a.D + (size_t)vv[k] * a.S + col);
 }
 
 // Issues the atomicMins of the lanes whose candidate beats the loaded distance.
-__device__ __forceinline__ void window_atomics(const MArgs &a, const LaneSched &ls, const U32xG &vv, Window &w) {
+__device__ __forceinline__ void window_atomics(const MArgs &a, const U32xG &vv, uint32_t col, Window &w) {
 #pragma unroll
-  for (int k = 0; k < kGroup; ++k) w.val[k] = w.nd[k] < w.val[k] ? atomic_min(cell(a, ls, vv[k]), w.nd[k]) : 0u;
+  for (int k = 0; k < kGroup; ++k)
+    w.val[k] = w.nd[k] < w.val[k] ? atomic_min(a.D + (size_t)vv[k] * a.S + col, w.nd[k]) : 0u;
 }
 
 // Takes a window's atomic results (issued earlier, so the wait overlaps the
@@ -350,10 +316,9 @@ __device__ __forceinline__ void window_results(const MArgs &a, LaneSched &ls, Wi
 __device__ __forceinline__ void relax_edges(const MArgs &a, const Sched &s, LaneSched &ls, uint32_t du, uint32_t lo,
                                             uint32_t hi, Pending &p, uint64_t &improved) {
   const uint32_t lane = lane_id();
-  // the planes this item's sources need (lanes without a candidate, du = INF,
-  // load a word of a needed plane's row and never issue an atomic)
-  const uint64_t cm = __ballot(du != kInf);
-  const bool needA = (uint32_t)cm != 0, needB = (cm >> 32) != 0;
+  // lanes >= S have no source and du = INF: they load the row's last word
+  // and never issue an atomic
+  const uint32_t col = min(lane, a.S - 1);
   // the bounds are wave-uniform; say so, so that they stay in scalar registers
   lo = __builtin_amdgcn_readfirstlane(lo);
   hi = __builtin_amdgcn_readfirstlane(hi);
@@ -362,7 +327,7 @@ __device__ __forceinline__ void relax_edges(const MArgs &a, const Sched &s, Lane
   Window cur, prev;
   bool have_prev = false;
   for (;;) {
-    window_load(a, ls, du, vv, ww, e - w0, needA, needB, cur);
+    window_load(a, du, vv, ww, e - w0, col, cur);
     cur.dst = lane < kGroup ? glb(a.edge_dst)[w0 + lane] : 0u;
     const uint32_t en = e + kGroup;
     const bool more = en < hi;
@@ -370,7 +335,7 @@ __device__ __forceinline__ void relax_edges(const MArgs &a, const Sched &s, Lane
     const U32xG vvn = scalar_window(a.edge_dst, wn), wwn = scalar_window(a.edge_data, wn);
     if (have_prev) window_results(a, ls, prev, p, improved);
     if (p.n > (uint32_t)(kWave - kGroup)) flush_pending(a, s, ls, p);
-    window_atomics(a, ls, vv, cur);
+    window_atomics(a, vv, col, cur);
     prev = cur;
     have_prev = true;
     if (!more) break;
@@ -431,7 +396,7 @@ __device__ __forceinline__ void process_items(const MArgs &a, const Sched &s, La
   for (int k = 0; k < kItems; ++k) {
     const uint64_t mk = bcast64(m, k);
     const uint32_t uk = readlane(u, k);
-    du[k] = lane < a.S && ((mk >> lane) & 1) ? ld_coh(cell(a, ls, uk)) : kInf;
+    du[k] = lane < a.S && ((mk >> lane) & 1) ? ld_coh(a.D + (size_t)uk * a.S + lane) : kInf;
   }
   const bool split = s.aux[0] && s.aux[1];
   uint32_t mydeg = 0;  // lane k: edges of item k to relax in the flattened list
@@ -512,7 +477,7 @@ __device__ __forceinline__ void process_chunk(const MArgs &a, const Sched &s, La
   if ((uint64_t)k * a.heavy_deg >= re - rs) return;
   uint32_t lo = rs + k * a.heavy_deg, hi = min(re, lo + a.heavy_deg);
   bool act = lane < a.S && ((m >> lane) & 1);
-  uint32_t du = act ? ld_coh(cell(a, ls, u)) : kInf;
+  uint32_t du = act ? ld_coh(a.D + (size_t)u * a.S + lane) : kInf;
   act = act && du != kInf;
   uint64_t am = __ballot(act);
   if (am == 0) return;
@@ -678,7 +643,6 @@ void multi_kernel(MArgs a, MultiCounters *tot) {
     ls.fmin = kInf;
     ls.pb = &pushbuf[threadIdx.x / kWave][0];
     ls.pbn[0] = ls.pbn[1] = 0;
-    lane_cols(a, ls);
     Pending p;
     p.n = 0;
     p.v = p.nb_lo = p.nb_hi = p.fb_lo = p.fb_hi = 0;
@@ -735,9 +699,8 @@ void multi_kernel(MArgs a, MultiCounters *tot) {
 // D[v][s] = INF, masks cleared.
 __global__ void multi_init(MArgs a, MultiCounters *tot) {
   size_t tid = blockIdx.x * (size_t)blockDim.x + threadIdx.x, nt = (size_t)gridDim.x * blockDim.x;
-  const size_t ca = (size_t)a.nnodes << a.shA, cb = a.S > 32 ? (size_t)a.nnodes << a.shB : 0;
-  for (size_t i = tid; i < ca; i += nt) a.DA[i] = kInf;
-  for (size_t i = tid; i < cb; i += nt) a.DB[i] = kInf;
+  size_t cells = (size_t)a.nnodes * a.S;
+  for (size_t i = tid; i < cells; i += nt) a.D[i] = kInf;
   for (size_t v = tid; v < a.nnodes; v += nt) {
     a.nmask[0][v] = 0;
     a.nmask[1][v] = 0;
@@ -755,10 +718,8 @@ __global__ void multi_seed(MArgs a) {
   bool act = lane < a.S;
   uint32_t src = act ? a.srcs[lane] : 0u;
   bool push = false;
-  LaneSched ls;
-  lane_cols(a, ls);
   if (act) {
-    *cell(a, ls, src) = 0u;
+    a.D[(size_t)src * a.S + lane] = 0u;
     push = atomic_or64(a.nmask[0] + src, 1ull << lane) == 0;
     a.qstamp[src] = 1u;  // in round 1's queue
   }
@@ -812,36 +773,18 @@ __global__ void source_degree_kernel(const uint32_t *row_start, const uint32_t *
   if (threadIdx.x < n) deg[threadIdx.x] = row_start[srcs[threadIdx.x] + 1] - row_start[srcs[threadIdx.x]];
 }
 
-// Row shape of the two-plane matrix of S sources: plane A rows of 2^shA
-// words (sources 0..31), plane B rows of 2^shB words (sources 32..S-1).
-static void plane_shape(uint32_t S, uint32_t *shA, uint32_t *shB) {
-  uint32_t a = 0, b = 0;
-  while ((1u << a) < std::min(S, 32u)) ++a;
-  while (S > 32 && (1u << b) < S - 32) ++b;
-  *shA = a;
-  *shB = b;
-}
-static size_t plane_words(uint32_t V, uint32_t S) {
-  uint32_t shA, shB;
-  plane_shape(S, &shA, &shB);
-  return ((size_t)V << shA) + (S > 32 ? (size_t)V << shB : 0);
-}
-
-// The caller's D [V][S] from a launch's two-plane matrix of the sources that
-// have out-edges: column s is plane column col[s], or, for a source without
-// out-edges, INF except 0 at the source itself.  One wavefront per row, lane
-// = column (8 rows per wavefront step measured slower: 0.63 vs 0.48 ms on
-// RMAT-22).
-__global__ void multi_expand(const uint32_t *DA, const uint32_t *DB, uint32_t shA, uint32_t shB, const int32_t *col,
-                             const uint32_t *srcs, uint32_t S, uint32_t V, uint32_t *D) {
+// D [V][S] from the compact Dc [V][Sc] of the sources that have out-edges:
+// column s is Dc's column col[s], or, for a source without out-edges, INF
+// except 0 at the source itself.  One wavefront per row, lane = column (8
+// rows per wavefront step measured slower: 0.63 vs 0.48 ms on RMAT-22).
+__global__ void multi_expand(const uint32_t *Dc, uint32_t Sc, const int32_t *col, const uint32_t *srcs, uint32_t S,
+                             uint32_t V, uint32_t *D) {
   const uint32_t lane = lane_id();
   const size_t nwaves = (size_t)gridDim.x * blockDim.x / kWave;
   const int32_t c = lane < S ? col[lane] : -1;
   const uint32_t sv = lane < S ? srcs[lane] : 0u;
-  const uint32_t *base = c < 32 ? DA + c : DB + (c - 32);
-  const uint32_t sh = c < 32 ? shA : shB;
   for (size_t v = (blockIdx.x * (size_t)blockDim.x + threadIdx.x) / kWave; v < V; v += nwaves)
-    if (lane < S) D[v * S + lane] = c >= 0 ? base[v << sh] : (v == sv ? 0u : kInf);
+    if (lane < S) D[v * S + lane] = c >= 0 ? Dc[v * Sc + (uint32_t)c] : (v == sv ? 0u : kInf);
 }
 
 using MultiFn = void (*)(MArgs, MultiCounters *);
@@ -860,10 +803,9 @@ struct MultiRun {
   int grid = 0, bs = 0;
 };
 
-// Runs one vertex-major launch: S = nsrc <= 64 sources of `g` into the
-// two-plane matrix at `planes` (plane_words(V, S) words).
+// Runs one vertex-major launch: S = nsrc <= 64 sources of `g` into D [V][S].
 static int run_multi(Workspace *w, const lsg_graph *g, const uint32_t *srcs_host, uint32_t nsrc, const lsg_opts &o,
-                     uint32_t *planes, MultiRun *out) {
+                     uint32_t *D, MultiRun *out) {
   int rc;
   int bs = o.block_size ? o.block_size : 512;
   if (bs != 256 && bs != 512 && bs != 1024) return fail(LSG_ERR_ARG, "block_size must be 256, 512 or 1024");
@@ -883,9 +825,7 @@ static int run_multi(Workspace *w, const lsg_graph *g, const uint32_t *srcs_host
   a.row_start = g->row_start;
   a.edge_dst = g->edge_dst;
   a.edge_data = (const uint32_t *)g->edge_data;
-  plane_shape(nsrc, &a.shA, &a.shB);
-  a.DA = planes;
-  a.DB = nsrc > 32 ? planes + ((size_t)g->nnodes << a.shA) : nullptr;
+  a.D = D;
   uint64_t *masks = (uint64_t *)mbuf;
   a.nmask[0] = masks;
   a.nmask[1] = masks + V;
@@ -994,11 +934,9 @@ static int run_multi(Workspace *w, const lsg_graph *g, const uint32_t *srcs_host
 
 // Runs nsrc <= 64 sources into D [V][nsrc].  A source without out-edges
 // reaches only itself, so it gets no lane: the launch runs the other sources
-// into a compact two-plane matrix (on RMAT-22, where half the vertices have
-// no out-edges, 64 random sources leave 33 lanes: 25.5 -> 23.6 ms with rows
-// of 132 instead of 256 bytes), which is then expanded.  When every source
-// has out-edges and their count is a power of two up to 32, the plane is
-// the caller's matrix itself.
+// into a compact matrix (rows of fewer words: on RMAT-22, where half the
+// vertices have no out-edges, 64 random sources leave 33 lanes and rows of
+// 132 instead of 256 bytes, 25.5 -> 23.6 ms), which is then expanded.
 static int run_multi_live(Workspace *w, const lsg_graph *g, const uint32_t *srcs, uint32_t nsrc, const lsg_opts &o,
                           uint32_t *D, MultiRun *out) {
   int rc;
@@ -1019,14 +957,12 @@ static int run_multi_live(Workspace *w, const lsg_graph *g, const uint32_t *srcs
     col[i] = deg[i] ? (int32_t)nlive : -1;
     if (deg[i]) live[nlive++] = srcs[i];
   }
-  if (nlive == nsrc && nsrc <= 32 && (nsrc & (nsrc - 1)) == 0) return run_multi(w, g, srcs, nsrc, o, D, out);
+  if (nlive == nsrc) return run_multi(w, g, srcs, nsrc, o, D, out);
   const uint32_t V = g->nnodes;
   void *cbuf = nullptr;
-  uint32_t shA = 0, shB = 0;
   if (nlive) {
-    if ((rc = ws_buffer(w, kBufTranspose, plane_words(V, nlive) * 4, &cbuf))) return rc;
+    if ((rc = ws_buffer(w, kBufTranspose, (size_t)V * nlive * 4, &cbuf))) return rc;
     if ((rc = run_multi(w, g, live, nlive, o, (uint32_t *)cbuf, out))) return rc;
-    plane_shape(nlive, &shA, &shB);
   } else {
     *out = MultiRun();
     w->last_trace.clear();
@@ -1034,10 +970,8 @@ static int run_multi_live(Workspace *w, const lsg_graph *g, const uint32_t *srcs
   }
   HIPCHK(hipMemcpyAsync(d_col, col, nsrc * 4, hipMemcpyHostToDevice, w->stream));
   const uint32_t blocks = (uint32_t)std::min<uint64_t>(((uint64_t)V * kWave + 255) / 256, 16ull * w->cus);
-  const uint32_t *DA = (const uint32_t *)cbuf;
-  const uint32_t *DB = nlive > 32 ? DA + ((size_t)V << shA) : DA;
-  hipLaunchKernelGGL(multi_expand, dim3(blocks), dim3(256), 0, w->stream, DA, DB, shA, shB, (const int32_t *)d_col,
-                     (const uint32_t *)d_srcs, nsrc, V, D);
+  hipLaunchKernelGGL(multi_expand, dim3(blocks), dim3(256), 0, w->stream, (const uint32_t *)cbuf, nlive,
+                     (const int32_t *)d_col, (const uint32_t *)d_srcs, nsrc, V, D);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(w->stream));
   return LSG_OK;

@@ -115,7 +115,6 @@ __global__ __launch_bounds__(kLbBlock) void lb_relax_kernel(KArgs a, LbLists l) 
     sched_load(ss, &a.ctrl->s);
     sh.ps.local = 0;
     sh.ps.far_min = kInf;
-    sh.ps.in_lds = 0;
   }
   __syncthreads();
   const Sched s = ss;
@@ -160,7 +159,6 @@ __global__ __launch_bounds__(BS) void lb_sweep_kernel(KArgs a) {
     sched_load(ss, &c->s);
     sh.ps.local = 0;
     sh.ps.far_min = kInf;
-    sh.ps.in_lds = 0;
     sh.edges = 0;
   }
   __syncthreads();

@@ -65,52 +65,10 @@ struct KArgs {
 // serialises at the memory-side atomic unit (~12 ns each on gfx950).
 constexpr uint32_t kStage = 256;
 
-// The near pile of block-local rounds, kept in LDS: a table of (vertex,
-// smallest near candidate) with the list of its used slots in insertion
-// order, one table per round parity (filled in round r, consumed in r + 1).
-// The table replaces the global near stamps and queue while one block runs
-// rounds on its own: a round then costs the row, edge and atomicMin round
-// trips of its relaxations and no queue or stamp traffic.
-constexpr uint32_t kLocalHash = 1024;  // table slots (power of two)
-constexpr uint32_t kLocalCap = 512;    // listed entries; more set the overflow flag
-constexpr uint32_t kLocalProbe = 32;   // linear-probe bound of an insertion
-struct LocalNear {
-  uint32_t key[2][kLocalHash];   // vertex + 1, 0 = empty
-  uint32_t dist[2][kLocalHash];  // smallest candidate inserted (INF when empty)
-  uint16_t list[2][kLocalCap];   // used slots, insertion order
-  uint32_t n[2];                 // slots claimed (may exceed kLocalCap)
-  uint32_t overflow;             // an insertion found no slot or no list entry this round
-};
-
-// Inserts v with candidate nd into table p; false when the probe path is full
-// (the caller then pushes v to the global near pile).
-__device__ __forceinline__ bool local_insert(LocalNear &ln, uint32_t p, uint32_t v, uint32_t nd) {
-  uint32_t h = (v * 2654435761u) >> 22;  // top 10 bits: kLocalHash = 1024
-  for (uint32_t k = 0; k < kLocalProbe; ++k, h = (h + 1) & (kLocalHash - 1)) {
-    const uint32_t key = atomicCAS(&ln.key[p][h], 0u, v + 1);
-    if (key == 0) {
-      const uint32_t i = atomicAdd(&ln.n[p], 1u);
-      if (i < kLocalCap) ln.list[p][i] = (uint16_t)h;
-      else ln.overflow = 1;
-    }
-    if (key == 0 || key == v + 1) {
-      atomicMin(&ln.dist[p][h], nd);
-      return true;
-    }
-  }
-  ln.overflow = 1;
-  return false;
-}
-
 template <int BS>
 struct PushStage {
   uint32_t item[BS / kWave][2][kStage];  // [wave][near, far]
   uint32_t n[BS / kWave][2];
-  // Block-local rounds: near items (without stamps) and their candidates,
-  // staged for the LDS table.
-  uint32_t litem[BS / kWave][kStage];
-  uint32_t lcand[BS / kWave][kStage];
-  uint32_t ln_n[BS / kWave];
   // Bucket far mode: smallest distance this block sent to the far pile this
   // round (pushed candidates and deferred items), folded into the control
   // block's far_min once per round.
@@ -119,29 +77,7 @@ struct PushStage {
   // tails are kept here and stored once at the end of the round.
   uint32_t local;
   uint32_t tail[2];
-  // Block-local rounds: near pushes go to table near_par of ln; in_lds = the
-  // round's in-list is table in_par's list.
-  uint32_t near_par, in_lds, in_par;
-  uint32_t heavy_seen;  // a heavy vertex was queued as chunks this round
-  LocalNear ln;
 };
-
-// Every block clears its tables once at kernel start (only the block that
-// runs local rounds uses them).
-template <int BS>
-__device__ __forceinline__ void local_init(PushStage<BS> &ps) {
-  for (uint32_t i = threadIdx.x; i < 2 * kLocalHash; i += BS) {
-    (&ps.ln.key[0][0])[i] = 0u;
-    (&ps.ln.dist[0][0])[i] = kInf;
-  }
-  if (threadIdx.x < BS / kWave) ps.ln_n[threadIdx.x] = 0;
-  if (threadIdx.x == 0) {
-    ps.ln.n[0] = ps.ln.n[1] = 0;
-    ps.ln.overflow = 0;
-    ps.in_lds = 0;
-    ps.heavy_seen = 0;
-  }
-}
 
 // Writes this wavefront's staged items of pile `which` (0 near, 1 far) to the queue.
 template <int BS>
@@ -161,28 +97,6 @@ __device__ __forceinline__ void stage_flush(const KArgs &a, const Sched &s, Push
   ps.n[w][which] = 0;
 }
 
-// End of a block-local round: the wavefront's staged near items go into the
-// LDS table, which keeps each vertex once with its smallest candidate; an item
-// whose probe path is full goes to the global near pile instead, deduplicated
-// by the near stamps as in a grid round.
-template <int BS>
-__device__ __forceinline__ void local_absorb(const KArgs &a, const Sched &s, PushStage<BS> &ps) {
-  const uint32_t w = threadIdx.x / kWave;
-  const uint32_t n = __builtin_amdgcn_readfirstlane(ps.ln_n[w]);
-  const uint32_t stamp = s.round + 1;
-  for (uint32_t i0 = 0; i0 < n; i0 += kWave) {
-    const uint32_t i = i0 + lane_id();
-    bool spill = false;
-    uint32_t v = 0;
-    if (i < n) {
-      v = ps.litem[w][i];
-      spill = !local_insert(ps.ln, ps.near_par, v, ps.lcand[w][i]) && atomic_exch(a.near_stamp + v, stamp) != stamp;
-    }
-    stage_push<BS>(a, s, ps, 0, spill, v);
-  }
-  ps.ln_n[w] = 0;
-}
-
 // Pushes `item` from every lane with `pred` onto pile `which`.
 template <int BS>
 __device__ __forceinline__ void stage_push(const KArgs &a, const Sched &s, PushStage<BS> &ps, int which, bool pred,
@@ -198,32 +112,6 @@ __device__ __forceinline__ void stage_push(const KArgs &a, const Sched &s, PushS
   }
   if (pred) ps.item[w][which][n + lanes_below(m)] = item;
   ps.n[w][which] = n + c;
-}
-
-// A near push of a block-local round: the item and its candidate, staged
-// for the LDS table.  When the wavefront's local stage is full, the items
-// take the grid path instead (near stamps, the global near pile) and the
-// block-local run ends after this round.
-template <int BS>
-__device__ __forceinline__ void stage_push_local(const KArgs &a, const Sched &s, PushStage<BS> &ps, bool pred,
-                                                 uint32_t item, uint32_t cand) {
-  const uint64_t m = __ballot(pred);
-  if (m == 0) return;
-  const uint32_t w = threadIdx.x / kWave;
-  const uint32_t c = (uint32_t)__builtin_popcountll(m);
-  const uint32_t n = __builtin_amdgcn_readfirstlane(ps.ln_n[w]);
-  if (n + c <= kStage) {
-    if (pred) {
-      ps.litem[w][n + lanes_below(m)] = item;
-      ps.lcand[w][n + lanes_below(m)] = cand;
-    }
-    ps.ln_n[w] = n + c;
-    return;
-  }
-  if (lane_id() == 0) ps.ln.overflow = 1;
-  const uint32_t stamp = s.round + 1;
-  const bool push = pred && atomic_exch(a.near_stamp + item, stamp) != stamp;
-  stage_push<BS>(a, s, ps, 0, push, item);
 }
 
 // End of a round: every wavefront writes out what it staged.
@@ -259,9 +147,8 @@ __device__ __forceinline__ void flush_far_min(const KArgs &a, PushStage<BS> &ps)
 // really lowered d[v] (the reference pushes on the pre-check alone), and the
 // near/far split at curdelta (:134).  Arithmetic is u32: d[u] < 2^31 and
 // w <= INT_MAX never wrap, and a candidate >= INT_MAX never beats INF.
-// L: a block-local round, whose near pushes go to the LDS table.
 struct SsspRelax {
-  template <int BS, bool L>
+  template <int BS>
   __device__ __forceinline__ static void edge(const KArgs &a, const Sched &s, uint32_t v, uint32_t nd, bool active,
                                               PushStage<BS> &ps, bool precheck) {
     // The pre-checks read through the cache: a stale d[v] is only ever larger
@@ -273,21 +160,17 @@ struct SsspRelax {
     const bool near = (int64_t)nd <= s.curdelta();
     uint32_t *stamps = near ? a.near_stamp : a.far_stamp;
     const uint32_t stamp = near ? s.round + 1 : s.phase + 1;
-    // block-local rounds keep near pushes in the LDS table instead
-    const bool lds = L && near;
     uint32_t seen = stamp;
     if (active) {
       const uint32_t dv = precheck ? glb(a.dist)[v] : kInf;
       // the stamp is loaded beside d[v]: one round trip fewer when the edge
       // improves (C1 12.3 -> 11.9 ms, C3 12.0 -> 11.9 ms, RMAT unchanged)
-      seen = lds ? 0u : glb(stamps)[v];
+      seen = glb(stamps)[v];
       if (!precheck || nd < dv) imp = nd < atomic_min(a.dist + v, nd);
     }
     // Push once per round (near) or phase (far): the exchange decides between
     // racing improvers, the cached pre-check skips it when already stamped.
-    // A block-local round stages its near pushes for the LDS table instead.
-    const bool push = imp && !lds && seen != stamp && atomic_exch(stamps + v, stamp) != stamp;
-    if (L) stage_push_local<BS>(a, s, ps, imp && lds, v, nd);
+    const bool push = imp && seen != stamp && atomic_exch(stamps + v, stamp) != stamp;
     const bool to_near = push && near, to_far = push && !near;
     if (s.aux[1] == LSG_FAR_BUCKET) note_far_min<BS>(ps, imp && !near, nd);
     stage_push<BS>(a, s, ps, 0, to_near, v);
@@ -301,17 +184,13 @@ struct SsspRelax {
 // one round after their vertex is popped) still ends at its hop count; every
 // improvement is pushed to the next level's pile, once per round.
 struct BfsVisit {
-  template <int BS, bool L>
+  template <int BS>
   __device__ __forceinline__ static void edge(const KArgs &a, const Sched &s, uint32_t v, uint32_t nd, bool active,
                                               PushStage<BS> &ps, bool precheck) {
     bool imp = false;
     if (active) {
       const uint32_t dv = precheck ? glb(a.dist)[v] : kInf;
       if (!precheck || nd < dv) imp = nd < atomic_min(a.dist + v, nd);
-    }
-    if (L) {
-      stage_push_local<BS>(a, s, ps, imp, v, nd);
-      return;
     }
     if (imp) {
       uint32_t stamp = s.round + 1;
@@ -321,7 +200,7 @@ struct BfsVisit {
   }
 };
 
-template <Algo A, int BS, bool L = false>
+template <Algo A, int BS>
 __device__ __forceinline__ void relax_edge(const KArgs &a, const Sched &s, uint32_t e, uint32_t du, bool active,
                                            PushStage<BS> &ps, bool precheck) {
   uint32_t v = 0, nd = 0;
@@ -329,8 +208,8 @@ __device__ __forceinline__ void relax_edge(const KArgs &a, const Sched &s, uint3
     v = glb(a.edge_dst)[e];
     nd = A == Algo::SSSP ? du + glb(a.edge_data)[e] : du + 1u;
   }
-  if (A == Algo::SSSP) SsspRelax::edge<BS, L>(a, s, v, nd, active, ps, precheck);
-  else BfsVisit::edge<BS, L>(a, s, v, nd, active, ps, precheck);
+  if (A == Algo::SSSP) SsspRelax::edge<BS>(a, s, v, nd, active, ps, precheck);
+  else BfsVisit::edge<BS>(a, s, v, nd, active, ps, precheck);
 }
 
 // ---- block-level helpers ---------------------------------------------------
@@ -382,27 +261,17 @@ struct TileShared {
 // each popped vertex (the reference's pop + filter, apps/sssp/kernel.cu:201-210),
 // queues heavy vertices as chunks for the next round, and expands the rest by
 // the selected mode.
-template <Algo A, int BS, bool L = false>
+template <Algo A, int BS>
 __device__ __forceinline__ void process_tile(const KArgs &a, const Sched &s, uint32_t t, uint32_t ti, TileShared<BS> &sh,
                              bool precheck = true) {
   uint32_t i = t * ti + threadIdx.x;
   bool valid = threadIdx.x < ti && i < s.in_count;
-  uint32_t u = 0, du = 0, rs = 0, re = 0, deg = 0;
-  if (L && sh.ps.in_lds) {
-    // a block-local round's in-list: the LDS table's entries, with their
-    // exact distances (only this block has lowered them since)
-    if (valid) {
-      const uint32_t h = sh.ps.ln.list[sh.ps.in_par][i];
-      u = sh.ps.ln.key[sh.ps.in_par][h] - 1;
-      du = sh.ps.ln.dist[sh.ps.in_par][h];
-    }
-  } else {
-    u = valid ? ld_coh(a.q[s.in_q] + i) : 0u;
-    if (valid && u < a.nnodes) du = ld_coh(a.dist + u);
-  }
+  uint32_t u = valid ? ld_coh(a.q[s.in_q] + i) : 0u;
+  uint32_t du = 0, rs = 0, re = 0, deg = 0;
   valid = valid && u < a.nnodes;
   if (valid) {
     // distance and row bounds in one round trip
+    du = ld_coh(a.dist + u);
     rs = glb(a.row_start)[u];
     re = glb(a.row_start)[u + 1];
     if (du == kInf) valid = false;  // INF nodes are never expanded (apps/sssp/kernel.cu:202)
@@ -428,7 +297,6 @@ __device__ __forceinline__ void process_tile(const KArgs &a, const Sched &s, uin
   bool heavy = deg >= a.heavy_deg;
   uint32_t nch = heavy ? (deg + a.heavy_deg - 1) / a.heavy_deg : 0u;
   if (__ballot(heavy)) {
-    sh.ps.heavy_seen = 1;
     uint32_t hq = 1u - s.heavy_in;
     uint32_t base = wave_reserve(nch, &a.ctrl->hcount[hq]);
     uint64_t *dst = reinterpret_cast<uint64_t *>(a.hq[hq]);
@@ -472,7 +340,7 @@ __device__ __forceinline__ void process_tile(const KArgs &a, const Sched &s, uin
         ee = sh.start[lo] + (e - sh.offs[lo]);
         d0 = sh.du[lo];
       }
-      relax_edge<A, BS, L>(a, s, ee, d0, act, sh.ps, precheck);
+      relax_edge<A, BS>(a, s, ee, d0, act, sh.ps, precheck);
     }
     __syncthreads();
   } else {
@@ -483,7 +351,7 @@ __device__ __forceinline__ void process_tile(const KArgs &a, const Sched &s, uin
     uint32_t mx = my;
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, off));
-    for (uint32_t j = 0; j < mx; ++j) relax_edge<A, BS, L>(a, s, rs + j, du, j < my, sh.ps, precheck);
+    for (uint32_t j = 0; j < mx; ++j) relax_edge<A, BS>(a, s, rs + j, du, j < my, sh.ps, precheck);
     uint64_t med = __ballot(!small && deg > 0);
     while (med) {
       uint32_t l = (uint32_t)__builtin_ctzll(med);
@@ -491,7 +359,7 @@ __device__ __forceinline__ void process_tile(const KArgs &a, const Sched &s, uin
       uint32_t ws = readlane(rs, l), wd = readlane(deg, l), wu = readlane(du, l);
       for (uint32_t j = 0; j < wd; j += kWave) {
         uint32_t jj = j + lane_id();
-        relax_edge<A, BS, L>(a, s, ws + jj, wu, jj < wd, sh.ps, precheck);
+        relax_edge<A, BS>(a, s, ws + jj, wu, jj < wd, sh.ps, precheck);
       }
     }
   }
@@ -663,134 +531,33 @@ __device__ __forceinline__ uint32_t tile_items(uint32_t in_count, uint32_t nbloc
 constexpr uint32_t kLocalTiles = 1;
 
 // `ls` and `labort`: the schedule and abort flag the leader step just left in LDS.
-//
-// While a local round's near output fits one tile, has no heavy vertex and
-// found room in the LDS table, the next round takes its in-list from the
-// table and no word of the schedule, queues or statistics goes through
-// global memory: the round's bookkeeping (round number, statistics, trace
-// record) stays in LDS.  Otherwise, and at the end of a phase, the table's
-// vertices join the global near pile (with the near stamps, so that vertices
-// a full probe path sent there are not queued twice) and the leader step runs
-// as after a grid round.
 template <Algo A, int BS>
 __device__ __forceinline__ void local_rounds(const KArgs &a, TileShared<BS> &sh, Sched &ls, uint32_t &labort, uint32_t &lfar) {
-  __shared__ uint32_t go, stay;
-  __shared__ uint64_t acc_items, acc_rounds;
+  __shared__ uint32_t go;
   Ctrl *c = a.ctrl;
-  PushStage<BS> &ps = sh.ps;
   for (;;) {
     if (threadIdx.x == 0) {
       go = !ls.done && ls.heavy_count == 0 && ls.in_count <= kLocalTiles * BS && labort == kAbortNone;
       sh.edges = 0;
-      ps.local = go;
-      ps.tail[0] = 0;  // the near pile starts empty
-      ps.tail[1] = lfar;
-      ps.in_lds = 0;
-      acc_items = acc_rounds = 0;
+      sh.ps.local = go;
+      sh.ps.tail[0] = 0;  // the near pile starts empty
+      sh.ps.tail[1] = lfar;
     }
     __syncthreads();
     if (!go) return;
-    for (;;) {
-      const Sched s = ls;
-      const uint32_t par = (s.round + 1) & 1;
-      if (threadIdx.x == 0) {
-        ps.near_par = par;
-        ps.heavy_seen = 0;
-        ps.ln.overflow = 0;
-      }
-      __syncthreads();
-      for (uint32_t t = 0; t * BS < s.in_count; ++t) {
-        process_tile<A, BS, true>(a, s, t, BS, sh, false);
-        __syncthreads();
-      }
-      local_absorb<BS>(a, s, ps);  // each wavefront's staged near items into the table
-      if (ps.in_lds) {
-        // the consumed table is empty again for round r + 2
-        const uint32_t ip = ps.in_par;
-        for (uint32_t i = threadIdx.x; i < s.in_count; i += BS) {
-          const uint32_t h = ps.ln.list[ip][i];
-          ps.ln.key[ip][h] = 0u;
-          ps.ln.dist[ip][h] = kInf;
-        }
-        if (threadIdx.x == 0) ps.ln.n[ip] = 0;
-      }
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        const uint32_t n = ps.ln.n[par];
-        // (a full probe path or list sets overflow; only then did a near
-        // push go to the global pile)
-        const bool more = n > 0 && n <= BS && !ps.ln.overflow && !ps.heavy_seen;
-        stay = more;
-        if (more) {
-          // the leader step's bookkeeping of a round that refills the near
-          // pile, kept in LDS (apps/sssp/kernel.cu:370-372: retry2)
-          acc_items += s.in_count;
-          acc_rounds += 1;
-          if (a.trace && s.round - 1 < a.trace_cap) {
-            lsg_round_trace &r = a.trace[s.round - 1];
-            r.round = s.round;
-            r.phase = s.phase;
-            r.in_items = s.in_count;
-            r.heavy_chunks = 0;
-            r.near_out = n;
-            r.far_out = ps.tail[1];
-            r.heavy_out = 0;
-            r.curdelta_lo = s.curdelta_lo;
-            r.curdelta_hi = s.curdelta_hi;
-            r.delta = s.delta;
-            r.t_end = wall_clock64();
-          }
-          ls.round += 1;
-          ls.in_count = n;
-          ls.aux[0] = 0;
-          ps.in_lds = 1;
-          ps.in_par = par;
-          st_coh(&c->progress, s.round);
-        }
-      }
-      __syncthreads();
-      if (!stay) break;
-    }
-    // The table's vertices join the global near pile; its slots are cleared.
     const Sched s = ls;
-    const uint32_t par = (s.round + 1) & 1;
-    const uint32_t stamp = s.round + 1;
-
-    for (uint32_t h0 = 0; h0 < kLocalHash; h0 += BS) {
-      const uint32_t h = h0 + threadIdx.x;
-      const uint32_t key = ps.ln.key[par][h];
-      bool push = key != 0;
-      if (push) {
-        ps.ln.key[par][h] = 0u;
-        ps.ln.dist[par][h] = kInf;
-        push = atomic_exch(a.near_stamp + (key - 1), stamp) != stamp;
-      }
-      stage_push<BS>(a, s, ps, 0, push, key - 1);
+    for (uint32_t t = 0; t * BS < s.in_count; ++t) {
+      process_tile<A, BS>(a, s, t, BS, sh, false);
+      __syncthreads();
     }
-    if (threadIdx.x == 0) ps.ln.n[par] = 0;
-    if (ps.in_lds) {
-      // clear the other table too, in case its consumed list was not complete
-      for (uint32_t h = threadIdx.x; h < kLocalHash; h += BS) {
-        ps.ln.key[ps.in_par][h] = 0u;
-        ps.ln.dist[ps.in_par][h] = kInf;
-      }
-    }
-    stage_drain<BS>(a, s, ps);
+    stage_drain<BS>(a, s, sh.ps);
     __syncthreads();  // every wavefront has taken its places from the LDS tails
     if (threadIdx.x == 0) {
-      st_coh(&c->qcount[s.near_q], ps.tail[0]);
-      st_coh(&c->qcount[s.far_q], ps.tail[1]);
-      flush_far_min<BS>(a, ps);
+      st_coh(&c->qcount[s.near_q], sh.ps.tail[0]);
+      st_coh(&c->qcount[s.far_q], sh.ps.tail[1]);
+      flush_far_min<BS>(a, sh.ps);
       if ((a.sched || a.feat) && sh.edges)
         __hip_atomic_fetch_add(&c->stat_edges, (uint64_t)sh.edges, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (acc_rounds) {
-        // rounds run from the LDS table, before the leader step adds this one
-        st_coh64(&c->stat_items, ld_coh64(&c->stat_items) + acc_items);
-        st_coh64(&c->stat_rounds, ld_coh64(&c->stat_rounds) + acc_rounds);
-      }
-      sched_store(&c->s, ls);
-      ps.in_lds = 0;
-      ps.ln.n[0] = ps.ln.n[1] = 0;
     }
     // this block's pushes and counts are complete before the leader step reads them
     order_release();
@@ -816,7 +583,6 @@ __global__ __launch_bounds__(BS) void persistent_kernel(KArgs a) {
     sh.ps.local = 0;
     sh.ps.far_min = kInf;
   }
-  local_init<BS>(sh.ps);
   for (;;) {
     if (threadIdx.x == 0) {
       Sched s;
