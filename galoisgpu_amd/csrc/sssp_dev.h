@@ -262,7 +262,7 @@ struct TileShared {
 // queues heavy vertices as chunks for the next round, and expands the rest by
 // the selected mode.
 template <Algo A, int BS>
-__device__ __forceinline__ void process_tile(const KArgs &a, const Sched &s, uint32_t t, uint32_t ti, TileShared<BS> &sh,
+__device__ void process_tile(const KArgs &a, const Sched &s, uint32_t t, uint32_t ti, TileShared<BS> &sh,
                              bool precheck = true) {
   uint32_t i = t * ti + threadIdx.x;
   bool valid = threadIdx.x < ti && i < s.in_count;
@@ -368,7 +368,7 @@ __device__ __forceinline__ void process_tile(const KArgs &a, const Sched &s, uin
 // One heavy chunk: heavy_deg consecutive edges of one vertex, spread over
 // the block (replaces the -l edge-balanced TB_LB pass, apps/sssp/kernel.cu:78-149).
 template <Algo A, int BS>
-__device__ __forceinline__ void process_heavy(const KArgs &a, const Sched &s, uint32_t c, TileShared<BS> &sh) {
+__device__ void process_heavy(const KArgs &a, const Sched &s, uint32_t c, TileShared<BS> &sh) {
   uint64_t ent = ld_coh64(reinterpret_cast<const uint64_t *>(a.hq[s.heavy_in]) + c);
   uint32_t u = (uint32_t)ent, k = (uint32_t)(ent >> 32);
   if (u >= a.nnodes) return;
@@ -396,7 +396,7 @@ __device__ __forceinline__ void process_heavy(const KArgs &a, const Sched &s, ui
 // counters and the far pile's count for the next round (its near pile starts
 // empty), so rounds that the same block runs next need not load them again.
 template <Algo A>
-__device__ __forceinline__ void leader_step(const KArgs &a, Sched &s, uint32_t &abort_seen, uint32_t &far_tail) {
+__device__ void leader_step(const KArgs &a, Sched &s, uint32_t &abort_seen, uint32_t &far_tail) {
   __shared__ SchedFeatures f;
   __shared__ MlpScratch sc;
   __shared__ uint32_t decide, consumed, far_min;
@@ -532,7 +532,7 @@ constexpr uint32_t kLocalTiles = 1;
 
 // `ls` and `labort`: the schedule and abort flag the leader step just left in LDS.
 template <Algo A, int BS>
-__device__ __forceinline__ void local_rounds(const KArgs &a, TileShared<BS> &sh, Sched &ls, uint32_t &labort, uint32_t &lfar) {
+__device__ void local_rounds(const KArgs &a, TileShared<BS> &sh, Sched &ls, uint32_t &labort, uint32_t &lfar) {
   __shared__ uint32_t go;
   Ctrl *c = a.ctrl;
   for (;;) {
