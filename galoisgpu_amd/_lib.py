@@ -183,6 +183,7 @@ SIGNATURES = {
                               c_float, c_uint64, c_void_p]),
     "lsg_mlp_infer": (c_int, [c_int, c_void_p, c_void_p, c_uint32, c_void_p]),
     "lsg_workspace_release": (c_int, [c_int]),
+    "lsg_sssp_dist_noalloc": (c_int, [POINTER(LsgGraph), POINTER(LsgOpts), c_void_p, POINTER(LsgStats)]),
     "lsg_dev_malloc": (c_int, [c_int, c_size_t, POINTER(c_void_p)]),
     "lsg_dev_free": (c_int, [c_int, c_void_p]),
     "lsg_dev_copy": (c_int, [c_int, c_void_p, c_void_p, c_size_t, c_int]),

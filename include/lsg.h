@@ -206,6 +206,17 @@ int lsg_bfs_gg_main(lsg_graph *hg, lsg_graph *gg, const lsg_opts *opts, lsg_stat
  * several result arrays resident). */
 int lsg_sssp_dist(const lsg_graph *gg, const lsg_opts *opts, int32_t *dist_dev, lsg_stats *stats);
 int lsg_bfs_dist(const lsg_graph *gg, const lsg_opts *opts, int32_t *level_dev, lsg_stats *stats);
+/* No counterpart in the reference: lsg_sssp_dist for callers that cannot
+ * allocate device memory during a call (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  It allocates nothing:
+ * every buffer beyond the caller's graph and dist (control block, item and
+ * heavy-chunk queues, push stamps, seed) lives in one static device array of
+ * about 22 MB that each GPU holds until the process ends, so graphs are
+ * limited to 2^20 vertices and 2^24 edges (LSG_ERR_ARG beyond).  Calls share
+ * that array: calls that overlap on one GPU race.  Runs on the device's null
+ * stream; trace, graph_launch and enable_lb are rejected (they need
+ * workspace buffers).  Same distances as lsg_sssp_dist, bit for bit. */
+int lsg_sssp_dist_noalloc(const lsg_graph *gg, const lsg_opts *opts, int32_t *dist_dev, lsg_stats *stats);
 /* Connected components (apps/cc/kernel.cu:630-723), edges taken without
  * direction (weakly connected components).  label_dev[v] gets the smallest
  * vertex id of v's component; *ncomponents (may be NULL) the number of
