@@ -12,10 +12,21 @@
 // with the smallest id of its component -- the same partition, with one
 // deterministic label per component.
 //
-// Layout: label u32[V] (the parent pointer, node_data); a control word pair
-// (changed flag, component count).  One iteration is one hook pass over all
-// edges (16 consecutive edges per thread, their source found once by binary
-// search) followed by pointer jumping to the root for every vertex.
+// Layout: label u32[V] (the parent pointer, node_data); four control words
+// (changed flag, component count, done flag, passes).  One iteration is one
+// hook pass over all edges (16 consecutive edges per thread, their source
+// found once by binary search) followed by pointer jumping to the root for
+// every vertex.
+//
+// The fixpoint test stays on the device: the host enqueues passes in
+// batches, a one-thread step kernel after each pass sets the done flag when
+// the pass hooked nothing, and every kernel of a later pass returns at once
+// when it finds the flag set.  The host reads the flag once per batch
+// instead of once per pass (the reference copies its flag back after every
+// pass, apps/cc/kernel.cu:690-716).  The passes stay separate launches: a
+// launch boundary is what makes one pass's labels visible to every XCD's L2
+// for the next, which a persistent kernel would have to buy with an L2
+// invalidate in every block at every pass.
 
 #include <hip/hip_runtime.h>
 
@@ -30,11 +41,24 @@
 namespace lsg {
 
 constexpr uint32_t kHookRun = 16;  // edges per thread in a hook pass
+constexpr uint32_t kPassBatch = 4;  // passes enqueued per host check
+
+// control words
+constexpr int kCtlChanged = 0, kCtlCount = 1, kCtlDone = 2, kCtlPasses = 3;
 
 __global__ void cc_init_kernel(uint32_t *label, uint32_t V, uint32_t *ctl) {
   for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < V; v += (size_t)gridDim.x * blockDim.x)
     label[v] = (uint32_t)v;
-  if (blockIdx.x == 0 && threadIdx.x == 0) ctl[0] = ctl[1] = 0;
+  if (blockIdx.x == 0 && threadIdx.x < 4) ctl[threadIdx.x] = 0;
+}
+
+// Whether an earlier pass already reached the fixpoint (written by
+// cc_step_kernel in an earlier launch, so a plain load sees it).
+__device__ __forceinline__ bool cc_done(const uint32_t *ctl) {
+  __shared__ uint32_t s_done;
+  if (threadIdx.x == 0) s_done = ctl[kCtlDone];
+  __syncthreads();
+  return s_done != 0;
 }
 
 // Root of v: follows parents (every parent id is <= the child's, so the walk
@@ -53,6 +77,7 @@ __device__ __forceinline__ uint32_t find_root(const uint32_t *label, uint32_t v)
 // lowered a parent.
 __global__ void cc_hook_kernel(const uint32_t *row_start, const uint32_t *edge_dst, uint32_t V, uint32_t E,
                                uint32_t *label, uint32_t *ctl) {
+  if (cc_done(ctl)) return;
   const size_t nthreads = (size_t)gridDim.x * blockDim.x;
   bool changed = false;
   for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t * kHookRun < E; t += nthreads) {
@@ -84,11 +109,12 @@ __global__ void cc_hook_kernel(const uint32_t *row_start, const uint32_t *edge_d
   __syncthreads();
   if (__ballot(changed) && lane_id() == 0) s_changed = 1;
   __syncthreads();
-  if (threadIdx.x == 0 && s_changed) st_coh(ctl, 1u);
+  if (threadIdx.x == 0 && s_changed) st_coh(ctl + kCtlChanged, 1u);
 }
 
 // Pointer jumping: every vertex's parent becomes its root.
-__global__ void cc_jump_kernel(uint32_t *label, uint32_t V) {
+__global__ void cc_jump_kernel(uint32_t *label, uint32_t V, const uint32_t *ctl) {
+  if (cc_done(ctl)) return;
   for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < V; v += (size_t)gridDim.x * blockDim.x) {
     const uint32_t r = find_root(label, (uint32_t)v);
     if (ld_coh(label + v) != r) atomic_min(label + v, r);
@@ -106,10 +132,17 @@ __global__ void cc_count_kernel(const uint32_t *label, uint32_t V, uint32_t *ctl
   for (int d = kWave / 2; d >= 1; d >>= 1) n += __shfl_xor(n, d);
   if (lane_id() == 0 && n) atomicAdd(&s_n, n);
   __syncthreads();
-  if (threadIdx.x == 0 && s_n) atomic_add(ctl + 1, s_n);
+  if (threadIdx.x == 0 && s_n) atomic_add(ctl + kCtlCount, s_n);
 }
 
-__global__ void cc_clear_kernel(uint32_t *ctl) { ctl[0] = 0; }
+// After each pass (one thread): count the pass, and either clear the changed
+// flag for the next pass or, when the pass hooked nothing, raise done.
+__global__ void cc_step_kernel(uint32_t *ctl) {
+  if (ctl[kCtlDone]) return;
+  ctl[kCtlPasses] += 1;
+  if (ctl[kCtlChanged]) ctl[kCtlChanged] = 0;
+  else ctl[kCtlDone] = 1;
+}
 
 }  // namespace lsg
 
@@ -137,35 +170,37 @@ int lsg_cc(const lsg_graph *g, int32_t *label_dev, uint32_t *ncomponents, lsg_st
   const uint32_t hgrid = (uint32_t)std::min<uint64_t>(((uint64_t)E / kHookRun + 256) / 256, 16ull * w->cus);
   HIPCHK(hipEventRecord(w->ev0, w->stream));
   hipLaunchKernelGGL(cc_init_kernel, dim3(vgrid), dim3(256), 0, w->stream, label, V, ctl);
-  uint32_t iters = 0;
-  for (;;) {
-    ++iters;
-    hipLaunchKernelGGL(cc_hook_kernel, dim3(hgrid), dim3(256), 0, w->stream, g->row_start, g->edge_dst, V, E, label,
-                       ctl);
-    hipLaunchKernelGGL(cc_jump_kernel, dim3(vgrid), dim3(256), 0, w->stream, label, V);
+  uint32_t *c = w->pinned;
+  for (uint64_t enqueued = 0;;) {
+    for (uint32_t k = 0; k < kPassBatch; ++k) {
+      hipLaunchKernelGGL(cc_hook_kernel, dim3(hgrid), dim3(256), 0, w->stream, g->row_start, g->edge_dst, V, E,
+                         label, ctl);
+      hipLaunchKernelGGL(cc_jump_kernel, dim3(vgrid), dim3(256), 0, w->stream, label, V, ctl);
+      hipLaunchKernelGGL(cc_step_kernel, dim3(1), dim3(1), 0, w->stream, ctl);
+    }
     HIPCHK(hipGetLastError());
-    uint32_t *c = w->pinned;
-    HIPCHK(hipMemcpyAsync(c, ctl, 4, hipMemcpyDeviceToHost, w->stream));
+    enqueued += kPassBatch;
+    HIPCHK(hipMemcpyAsync(c, ctl + kCtlDone, 4, hipMemcpyDeviceToHost, w->stream));
     HIPCHK(hipStreamSynchronize(w->stream));
-    if (c[0] == 0) break;
+    if (c[0]) break;
     // A pass that changes anything lowers a label and labels never rise, so a
     // fixpoint always comes; this only guards against a broken invariant.
-    if (iters > V + 2) return fail(LSG_ERR_TIMEOUT, "cc: no fixpoint after %u passes", iters);
-    hipLaunchKernelGGL(cc_clear_kernel, dim3(1), dim3(1), 0, w->stream, ctl);
+    if (enqueued > (uint64_t)V + 2) return fail(LSG_ERR_TIMEOUT, "cc: no fixpoint after %llu passes",
+                                                (unsigned long long)enqueued);
   }
   hipLaunchKernelGGL(cc_count_kernel, dim3(vgrid), dim3(256), 0, w->stream, label, V, ctl);
   HIPCHK(hipEventRecord(w->ev1, w->stream));
-  uint32_t *c = w->pinned;
-  HIPCHK(hipMemcpyAsync(c, ctl, 8, hipMemcpyDeviceToHost, w->stream));
+  HIPCHK(hipMemcpyAsync(c, ctl, 16, hipMemcpyDeviceToHost, w->stream));
   HIPCHK(hipStreamSynchronize(w->stream));
+  const uint32_t iters = c[kCtlPasses];
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, w->ev0, w->ev1));
-  if (ncomponents) *ncomponents = c[1];
+  if (ncomponents) *ncomponents = c[kCtlCount];
   if (stats) {
     memset(stats, 0, sizeof *stats);
     stats->ms_kernel = ms;
     stats->iterations = iters;
-    stats->items = c[1];
+    stats->items = c[kCtlCount];
     stats->grid_blocks = (int32_t)hgrid;
     stats->block_size = 256;
     stats->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

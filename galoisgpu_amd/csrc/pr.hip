@@ -18,6 +18,23 @@
 // next); the reverse CSR from lsg_graph_transpose; a list of the vertices
 // with in-degree >= kWaveDeg, which a whole wavefront sums (the others take
 // one lane each).
+//
+// The convergence test stays on the device: the host enqueues iterations in
+// batches; after each iteration a one-thread step kernel counts it and sets a
+// done flag when no rank moved by more than epsilon, and every kernel of a
+// later iteration returns at once when it finds the flag set.  The host reads
+// the flag once per batch instead of once per iteration.  Iterations stay
+// separate launches: the launch boundary makes one iteration's contributions
+// visible to every XCD's L2 for the next iteration's gathers, which a
+// persistent kernel would have to buy with an L2 invalidate in every block at
+// every iteration.
+//
+// Why pull Jacobi and not the reference's residual push: a push sweep adds
+// to its out-neighbours' residuals with one float atomic per edge (measured
+// on gfx950: ~27 G random 4-byte atomics/s, against ~90 G gathers/s of the
+// pull sweep here), and float atomics make every sum's order, and so the
+// ranks' last bits, depend on the schedule; the pull sweep has no atomics and
+// one summation order.
 
 #include <hip/hip_runtime.h>
 
@@ -43,7 +60,23 @@ struct PrArgs {
   float alpha;
   uint32_t *maxdiff;                 // bits of the largest |p_next - p_cur| / p_next (non-negative floats order as u32)
   const float *p_cur;
+  const uint32_t *done;              // set once an earlier iteration converged
 };
+
+constexpr uint32_t kIterBatch = 8;  // iterations enqueued per host check
+
+// control words; the largest change is kept per iteration parity, so the step
+// kernel can clear the next iteration's word while this one's is read
+constexpr int kCtlHeavy = 0, kCtlMax = 1, kCtlDone = 3, kCtlIters = 4;
+
+// Whether an earlier iteration converged (written by pr_step_kernel in an
+// earlier launch, so a plain load sees it).
+__device__ __forceinline__ bool pr_done(const PrArgs &a) {
+  __shared__ uint32_t s_done;
+  if (threadIdx.x == 0) s_done = *a.done;
+  __syncthreads();
+  return s_done != 0;
+}
 
 __global__ void pr_init_kernel(const uint32_t *row_start, const uint32_t *in_row, uint32_t V, float alpha, float *p,
                                float *c, uint32_t *heavy, uint32_t *nheavy) {
@@ -88,6 +121,7 @@ __device__ __forceinline__ void pr_reduce_max(const PrArgs &a, float dmax) {
 
 // One lane per vertex of in-degree < kWaveDeg.
 __global__ void pr_light_kernel(PrArgs a) {
+  if (pr_done(a)) return;
   float dmax = 0.0f;
   const size_t nt = (size_t)gridDim.x * blockDim.x;
   for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < a.nnodes; v += nt) {
@@ -115,6 +149,7 @@ __global__ void pr_light_kernel(PrArgs a) {
 // One wavefront per vertex of in-degree >= kWaveDeg: 64 in-edges per step,
 // then a butterfly sum (the same order every iteration).
 __global__ void pr_heavy_kernel(PrArgs a) {
+  if (pr_done(a)) return;
   float dmax = 0.0f;
   const uint32_t lane = lane_id();
   const size_t waves = (size_t)gridDim.x * blockDim.x / kWave;
@@ -142,6 +177,15 @@ __global__ void pr_heavy_kernel(PrArgs a) {
 
 __global__ void pr_clear_kernel(uint32_t *w, uint32_t n) {
   if (threadIdx.x < n) w[threadIdx.x] = 0;
+}
+
+// After iteration `it` (one thread): count it, raise done when no rank moved
+// by more than epsilon, and clear the next iteration's change word.
+__global__ void pr_step_kernel(uint32_t *ctl, uint32_t it, float epsilon) {
+  if (ctl[kCtlDone]) return;
+  ctl[kCtlIters] = it + 1;
+  if (__uint_as_float(ctl[kCtlMax + (it & 1)]) <= epsilon) ctl[kCtlDone] = 1;
+  ctl[kCtlMax + ((it + 1) & 1)] = 0;
 }
 
 }  // namespace lsg
@@ -174,10 +218,10 @@ int lsg_pagerank(const lsg_graph *g, const lsg_graph *gt, float alpha, float eps
   float *p[2] = {(float *)buf, (float *)buf + V};
   float *c[2] = {(float *)buf + 2ull * V, (float *)buf + 3ull * V};
   uint32_t *heavy = (uint32_t *)buf + 4ull * V;
-  uint32_t *ctl = (uint32_t *)cbuf;  // [0] heavy count, [1] max change
+  uint32_t *ctl = (uint32_t *)cbuf;  // heavy count, max change (x2), done, iterations
   const uint32_t vgrid = (uint32_t)std::min<uint64_t>(((uint64_t)V + 255) / 256, 16ull * w->cus);
   HIPCHK(hipEventRecord(w->ev0, w->stream));
-  hipLaunchKernelGGL(pr_clear_kernel, dim3(1), dim3(64), 0, w->stream, ctl, 2u);
+  hipLaunchKernelGGL(pr_clear_kernel, dim3(1), dim3(64), 0, w->stream, ctl, 5u);
   hipLaunchKernelGGL(pr_init_kernel, dim3(vgrid), dim3(256), 0, w->stream, g->row_start, gt->row_start, V, alpha,
                      p[0], c[0], heavy, ctl);
   HIPCHK(hipGetLastError());
@@ -193,27 +237,30 @@ int lsg_pagerank(const lsg_graph *g, const lsg_graph *gt, float alpha, float eps
   a.nheavy = nheavy;
   a.nnodes = V;
   a.alpha = alpha;
-  a.maxdiff = ctl + 1;
+  a.done = ctl + kCtlDone;
   const uint32_t hgrid = (uint32_t)std::min<uint64_t>(((uint64_t)nheavy * kWave + 255) / 256 + 1, 16ull * w->cus);
-  int iters = 0, cur = 0;
-  float change = 0.0f;
-  while (iters < max_iterations) {
-    a.p_cur = p[cur];
-    a.c_cur = c[cur];
-    a.p_next = p[cur ^ 1];
-    a.c_next = c[cur ^ 1];
-    hipLaunchKernelGGL(pr_clear_kernel, dim3(1), dim3(64), 0, w->stream, ctl + 1, 1u);
-    hipLaunchKernelGGL(pr_light_kernel, dim3(vgrid), dim3(256), 0, w->stream, a);
-    if (nheavy) hipLaunchKernelGGL(pr_heavy_kernel, dim3(hgrid), dim3(256), 0, w->stream, a);
+  // Iteration `it` reads p[it & 1], c[it & 1] and writes the other pair.
+  for (uint32_t it = 0; it < (uint32_t)max_iterations;) {
+    const uint32_t end = std::min<uint32_t>((uint32_t)max_iterations, it + kIterBatch);
+    for (; it < end; ++it) {
+      a.p_cur = p[it & 1];
+      a.c_cur = c[it & 1];
+      a.p_next = p[(it & 1) ^ 1];
+      a.c_next = c[(it & 1) ^ 1];
+      a.maxdiff = ctl + kCtlMax + (it & 1);
+      hipLaunchKernelGGL(pr_light_kernel, dim3(vgrid), dim3(256), 0, w->stream, a);
+      if (nheavy) hipLaunchKernelGGL(pr_heavy_kernel, dim3(hgrid), dim3(256), 0, w->stream, a);
+      hipLaunchKernelGGL(pr_step_kernel, dim3(1), dim3(1), 0, w->stream, ctl, it, epsilon);
+    }
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(hc, ctl + 1, 4, hipMemcpyDeviceToHost, w->stream));
+    HIPCHK(hipMemcpyAsync(hc, ctl + kCtlDone, 4, hipMemcpyDeviceToHost, w->stream));
     HIPCHK(hipStreamSynchronize(w->stream));
-    ++iters;
-    cur ^= 1;
-    memcpy(&change, hc, 4);
-    if (change <= epsilon) break;
+    if (hc[0]) break;
   }
-  HIPCHK(hipMemcpyAsync(rank_dev, p[cur], (size_t)V * 4, hipMemcpyDeviceToDevice, w->stream));
+  HIPCHK(hipMemcpyAsync(hc, ctl + kCtlIters, 4, hipMemcpyDeviceToHost, w->stream));
+  HIPCHK(hipStreamSynchronize(w->stream));
+  const int iters = (int)hc[0];
+  HIPCHK(hipMemcpyAsync(rank_dev, p[iters & 1], (size_t)V * 4, hipMemcpyDeviceToDevice, w->stream));
   HIPCHK(hipEventRecord(w->ev1, w->stream));
   HIPCHK(hipEventSynchronize(w->ev1));
   float ms = 0;
