@@ -2,17 +2,31 @@
 
 The reference tunes one compile-time knob, BLKSIZE, with OpenTuner by
 rewriting a header, rebuilding and timing three runs (tuning/lsgtuner.py:
-68-214, tuning/sssptuner.py:7-16).  Here every knob is a runtime option, so
-the tuner times the real kernels directly: for the batched kernel it searches
-delta, far-pile mode, heavy-vertex chunk size and block size; for the plain
-kernel delta, expansion mode and block size.  Each candidate is timed by the
-library's own HIP events (best of `repeats`), results are written as JSON
-(``tuning/<name>.json``) and ``tuned_options`` reads them back.
+68-214, tuning/sssptuner.py:7-16, tuning/bfstuner.py:10-23), and keeps one
+result set per GPU generation (tuning/configs/{fermi,kepler,lsg2},
+tuning/lsgtuner.py:96-115).  Here every knob is a runtime option, so the
+tuner times the real kernels directly, and results are kept per device
+architecture: ``tuning/<arch>/<workload>.json`` (for example
+``tuning/gfx950/rmat22_64src.json``), read back by ``tuned_options``.
 
-Distances do not depend on any of these knobs; the tuner checks that every
-candidate's distances equal the first candidate's before accepting it.
+Workloads are the benchmark configurations, each with its own search space:
 
-Run on a GPU:  python -m galoisgpu_amd.autotune --graph rmat:22 --nsrc 64 --out galoisgpu_amd/tuning/rmat22_64src.json
+  grid256        plain SSSP, 256^2 grid, source 0, delta 32 (fixed): expansion
+                 mode, block size, heavy-chunk size, far-pile mode
+  rmat20         plain SSSP and BFS, RMAT-20 from the max-degree vertex, delta 32
+  road2048       plain SSSP, 2048^2 road-like grid, source 0, delta 10000
+  rmat22_64src   batched SSSP, RMAT-22, 64 uniform sources: delta, far-pile
+                 mode, heavy-chunk size, block size (and the plain kernel on 8
+                 of the sources)
+  union256       256 RMAT graphs of scales 10-16 in one launch: delta, expansion
+                 mode, block size, heavy-chunk size
+  rmat24_32src   batched SSSP, RMAT-24, 32 uniform sources
+
+Each candidate is timed by the library's own HIP events (best of `repeats`);
+distances do not depend on any knob, and the tuner checks every candidate's
+distances against the first candidate's before accepting it.
+
+Run on a GPU:  python -m galoisgpu_amd.autotune --workload all
 """
 from __future__ import annotations
 
@@ -21,11 +35,12 @@ import itertools
 import json
 import os
 import time
-from typing import Dict, List, Optional
+from typing import Callable, Dict, List, Optional
 
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
+TUNING = os.path.join(HERE, "tuning")
 
 BATCH_SPACE = {
     "delta": [32, 256, 1024, 1 << 30],
@@ -38,6 +53,24 @@ PLAIN_SPACE = {
     "expand_mode": [0, 1],
     "heavy_deg": [256, 512, 4096],
     "block_size": [256, 512],
+}
+# Single-source configurations with a fixed delta: launch knobs only.
+PLAIN_LAUNCH_SPACE = {
+    "expand_mode": [0, 1],
+    "block_size": [256, 512],
+    "heavy_deg": [512, 4096],
+    "far_mode": [0, 2],
+}
+BFS_SPACE = {
+    "expand_mode": [0, 1],
+    "block_size": [256, 512, 1024],
+    "heavy_deg": [256, 512, 4096],
+}
+UNION_SPACE = {
+    "delta": [32, 256, 1 << 30],
+    "expand_mode": [0, 1],
+    "block_size": [256, 512],
+    "heavy_deg": [512, 4096],
 }
 
 
@@ -60,6 +93,10 @@ def make_graph(spec: str, seed: int = 1):
     raise ValueError(f"unknown graph spec {spec}")
 
 
+def _sorted(results: List[dict]) -> List[dict]:
+    return sorted(results, key=lambda r: r["ms"])
+
+
 def tune_batched(dev, srcs, space=None, repeats: int = 2, log=print) -> List[dict]:
     from .apps import Options
     from .batch import DeviceArray, sssp_multi_into
@@ -79,33 +116,61 @@ def tune_batched(dev, srcs, space=None, repeats: int = 2, log=print) -> List[dic
             log(f"batched {cand}: {ms:.2f} ms")
     finally:
         D.close()
-    return sorted(results, key=lambda r: r["ms"])
+    return _sorted(results)
 
 
-def tune_plain(dev, srcs, space=None, repeats: int = 1, log=print) -> List[dict]:
-    from .apps import Options, sssp_into
+def tune_single(dev, srcs, space, fixed: Optional[dict] = None, bfs: bool = False, repeats: int = 1,
+                log=print) -> List[dict]:
+    """Single-source kernel (SSSP or BFS): total of the best-of-`repeats` times
+    over `srcs` for every candidate of `space` (merged with `fixed`)."""
+    from .apps import Options, bfs_into, sssp_into
     from .batch import DeviceArray
+    run = bfs_into if bfs else sssp_into
     V = dev.nnodes
     d = DeviceArray(V * 4, dev.device)
     results, ref = [], {}
     try:
-        for cand in grid(space or PLAIN_SPACE):
+        for cand in grid(space):
+            kw = dict(fixed or {})
+            kw.update(cand)
             total = 0.0
             for s in srcs:
-                ms = min(sssp_into(dev, d.ptr, Options(start_node=int(s), **cand))["ms_kernel"]
-                         for _ in range(repeats))
-                total += ms
+                total += min(run(dev, d.ptr, Options(start_node=int(s), **kw))["ms_kernel"] for _ in range(repeats))
                 h = d.to_numpy(np.int32, V)
                 key = int(s)
                 if key not in ref:
                     ref[key] = h
                 elif not np.array_equal(h, ref[key]):
-                    raise RuntimeError(f"plain candidate {cand} changed the distances of source {s}")
+                    raise RuntimeError(f"candidate {kw} changed the distances of source {s}")
             results.append(dict(cand, ms=total))
-            log(f"plain {cand}: {total:.2f} ms for {len(srcs)} sources")
+            log(f"{'bfs' if bfs else 'sssp'} {kw}: {total:.3f} ms for {len(srcs)} source(s)")
     finally:
         d.close()
-    return sorted(results, key=lambda r: r["ms"])
+    return _sorted(results)
+
+
+def tune_plain(dev, srcs, space=None, repeats: int = 1, log=print) -> List[dict]:
+    return tune_single(dev, srcs, space or PLAIN_SPACE, repeats=repeats, log=log)
+
+
+def tune_union(devs, srcs, space=None, repeats: int = 2, log=print) -> List[dict]:
+    from .apps import Options
+    from .batch import sssp_graphs
+    results, ref = [], None
+    for cand in grid(space or UNION_SPACE):
+        o = Options(**cand)
+        best, outs = None, None
+        for _ in range(repeats):
+            outs, st = sssp_graphs(devs, srcs, o)
+            best = st["ms_kernel"] if best is None else min(best, st["ms_kernel"])
+        h = np.concatenate(outs)
+        if ref is None:
+            ref = h
+        elif not np.array_equal(h, ref):
+            raise RuntimeError(f"union candidate {cand} changed the distances")
+        results.append(dict(cand, ms=best))
+        log(f"union {cand}: {best:.3f} ms")
+    return _sorted(results)
 
 
 def device_name(device: int = 0) -> str:
@@ -116,44 +181,111 @@ def device_name(device: int = 0) -> str:
     return buf.value.decode()
 
 
-def run(graph: str, nsrc: int, out: str, seed: int = 1, plain_sources: int = 8, log=print) -> dict:
-    g = make_graph(graph, seed)
-    rng = np.random.default_rng(1000 + seed)
-    srcs = rng.integers(0, g.nnodes, nsrc).astype(np.uint32)
-    dev = g.copy_to_gpu(0)
-    t0 = time.time()
-    batched = tune_batched(dev, srcs, log=log)
-    nontrivial = [s for s in srcs if g.row_start[s + 1] > g.row_start[s]][:plain_sources]
-    plain = tune_plain(dev, np.array(nontrivial, dtype=np.uint32), log=log)
-    dev.close()
-    res = {"graph": graph, "seed": seed, "nsrc": nsrc, "device": device_name(0),
-           "created": time.strftime("%Y-%m-%d %H:%M:%S"), "seconds": round(time.time() - t0, 1),
-           "batched": {k: v for k, v in batched[0].items() if k != "ms"}, "batched_ms": batched[0]["ms"],
-           "plain": {k: v for k, v in plain[0].items() if k != "ms"}, "plain_ms": plain[0]["ms"],
-           "plain_sources": [int(s) for s in nontrivial],
-           "batched_results": batched, "plain_results": plain}
-    os.makedirs(os.path.dirname(os.path.abspath(out)), exist_ok=True)
-    with open(out, "w") as f:
-        json.dump(res, f, indent=1)
-    log(f"best batched {res['batched']} {res['batched_ms']:.2f} ms; best plain {res['plain']} "
-        f"{res['plain_ms']:.2f} ms -> {out}")
+def device_arch(device: int = 0) -> str:
+    """'gfx950' from a device name such as 'AMD ... (gfx950:sramecc+:xnack-, 256 CUs)'."""
+    name = device_name(device)
+    if "(" in name:
+        return name.rsplit("(", 1)[1].split(":", 1)[0].split(",", 1)[0].strip()
+    return "unknown"
+
+
+def _best(results: List[dict]) -> dict:
+    return {k: v for k, v in results[0].items() if k != "ms"}
+
+
+def _union_graphs(n: int = 256, seed: int = 44):
+    from .graph import CSRGraph
+    rng = np.random.default_rng(seed)
+    scales = rng.integers(10, 17, n)
+    graphs = [CSRGraph.rmat(int(s), 8, 0.57, 0.19, 0.19, 1, 255, seed=100 + i) for i, s in enumerate(scales)]
+    return graphs, [g.max_degree_vertex() for g in graphs]
+
+
+def _tune_workload(name: str, log) -> dict:
+    """Runs the search of one workload; returns its result record."""
+    res: dict = {"workload": name}
+    if name in ("rmat22_64src", "rmat24_32src"):
+        # the sources bench.py and the C5 parity test use
+        scale, nsrc, rng_seed = (22, 64, 1001) if name == "rmat22_64src" else (24, 32, 1024)
+        g = make_graph(f"rmat:{scale}")
+        srcs = np.random.default_rng(rng_seed).integers(0, g.nnodes, nsrc).astype(np.uint32)
+        dev = g.copy_to_gpu(0)
+        batched = tune_batched(dev, srcs, log=log)
+        res.update(batched=_best(batched), batched_ms=batched[0]["ms"], batched_results=batched)
+        if scale == 22:
+            live = [s for s in srcs if g.row_start[s + 1] > g.row_start[s]][:8]
+            plain = tune_plain(dev, np.array(live, dtype=np.uint32), log=log)
+            res.update(plain=_best(plain), plain_ms=plain[0]["ms"], plain_sources=[int(s) for s in live],
+                       plain_results=plain)
+        dev.close()
+    elif name in ("grid256", "road2048", "rmat20"):
+        spec, delta = {"grid256": ("grid:256", 32), "road2048": ("road:2048", 10000), "rmat20": ("rmat:20", 32)}[name]
+        g = make_graph(spec)
+        src = g.max_degree_vertex() if name == "rmat20" else 0
+        dev = g.copy_to_gpu(0)
+        plain = tune_single(dev, [src], PLAIN_LAUNCH_SPACE, fixed={"delta": delta}, repeats=3, log=log)
+        res.update(plain=dict(_best(plain), delta=delta), plain_ms=plain[0]["ms"], plain_results=plain,
+                   source=int(src))
+        if name == "rmat20":
+            bfs = tune_single(dev, [src], BFS_SPACE, bfs=True, repeats=3, log=log)
+            res.update(bfs=_best(bfs), bfs_ms=bfs[0]["ms"], bfs_results=bfs)
+        dev.close()
+    elif name == "union256":
+        graphs, srcs = _union_graphs()
+        devs = [g.copy_to_gpu(0) for g in graphs]
+        union = tune_union(devs, srcs, log=log)
+        res.update(union=_best(union), union_ms=union[0]["ms"], union_results=union)
+        for d in devs:
+            d.close()
+    else:
+        raise ValueError(f"unknown workload {name}")
     return res
 
 
-def tuned_options(name: str, which: str) -> Optional[dict]:
-    """Best options of kind `which` ('batched' or 'plain') from tuning/<name>.json, or None."""
-    p = os.path.join(HERE, "tuning", f"{name}.json")
-    if not os.path.exists(p):
-        return None
-    with open(p) as f:
-        return dict(json.load(f)[which])
+WORKLOADS = ["grid256", "rmat20", "road2048", "rmat22_64src", "union256", "rmat24_32src"]
+
+
+def run(workload: str, out: Optional[str] = None, log=print, outdir: Optional[str] = None) -> dict:
+    """Tunes one workload on device 0 and writes <outdir>/<arch>/<workload>.json
+    (outdir defaults to the package's tuning/), or `out`."""
+    t0 = time.time()
+    res = _tune_workload(workload, log)
+    res.update(device=device_name(0), arch=device_arch(0), created=time.strftime("%Y-%m-%d %H:%M:%S"),
+               seconds=round(time.time() - t0, 1))
+    path = out or os.path.join(outdir or TUNING, res["arch"], f"{workload}.json")
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    with open(path, "w") as f:
+        json.dump(res, f, indent=1)
+    log(f"{workload}: {({k: res[k] for k in res if k in ('batched', 'plain', 'bfs', 'union')})} -> {path}")
+    return res
+
+
+def tuned_options(name: str, which: str, device: int = 0, root: Optional[str] = None) -> Optional[dict]:
+    """Best options of kind `which` ('batched', 'plain', 'bfs' or 'union') for
+    workload `name` on this device's architecture: tuning/<arch>/<name>.json,
+    else tuning/<name>.json; None when neither holds it."""
+    base = root or TUNING
+    paths = []
+    try:
+        paths.append(os.path.join(base, device_arch(device), f"{name}.json"))
+    except Exception:
+        pass
+    paths.append(os.path.join(base, f"{name}.json"))
+    for p in paths:
+        if os.path.exists(p):
+            with open(p) as f:
+                rec = json.load(f)
+            if which in rec:
+                return dict(rec[which])
+    return None
 
 
 if __name__ == "__main__":
-    ap = argparse.ArgumentParser(description="tune the persistent SSSP kernels on this GPU")
-    ap.add_argument("--graph", default="rmat:22")
-    ap.add_argument("--nsrc", type=int, default=64)
-    ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--out", required=True)
+    ap = argparse.ArgumentParser(description="tune the persistent SSSP / BFS kernels on this GPU")
+    ap.add_argument("--workload", default="rmat22_64src", help=f"one of {WORKLOADS} or 'all'")
+    ap.add_argument("--out", help="output file (default <outdir>/<arch>/<workload>.json)")
+    ap.add_argument("--outdir", help="directory of the per-architecture result sets (default: the package's tuning/)")
     a = ap.parse_args()
-    run(a.graph, a.nsrc, a.out, a.seed, log=lambda *x: print(*x, flush=True))
+    names = WORKLOADS if a.workload == "all" else [a.workload]
+    for n in names:
+        run(n, a.out if len(names) == 1 else None, log=lambda *x: print(*x, flush=True), outdir=a.outdir)
