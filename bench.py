@@ -3,9 +3,15 @@
 Workload (one "step"): SSSP from 64 sources sampled uniformly from a seeded
 RMAT scale-22 graph (edge factor 16, a/b/c/d = .57/.19/.19/.05, self loops and
 duplicates removed, weights uniform in [1,255]), delta 32.  The batched path
-runs the 64 sources in one persistent launch; the plain path runs them one
-after another through the single-source kernel.  Distances of the two are
-compared bit for bit before anything is timed.
+runs the 64 sources in one persistent launch, replayed as a captured hipGraph;
+the plain path runs them one after another through the single-source kernel.
+Distances of the two are compared bit for bit before anything is timed.
+
+Besides the headline line the JSON carries the learned schedule against the
+autotuned fixed one (median and spread of per-launch kernel times over the
+same number of launches each), and a with-allocation step: the library's
+cached device buffers are freed first, so that the step allocates them as the
+reference's gg_main allocates its worklists on every call.
 
 GTEPS = sum over sources of E_r (out-degree sum of reached vertices) / time.
 Algorithmic bytes per source B = 12 E_r + 16 V_r; roofline.achieved = sum B /
@@ -42,7 +48,8 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-plain", action="store_true", help="skip timing the plain kernel")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-sample", type=int, default=16, help="sources in the CPU baseline sample")
+    ap.add_argument("--cpu-sample", type=int, default=64,
+                    help="at most this many of the sources with out-edges in the CPU baseline sample")
     ap.add_argument("--sched", choices=["auto", "fixed", "learned"], default="auto")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -80,7 +87,8 @@ def main() -> int:
         use_learned = True
     # fixed batched schedule: the autotuner's choice (or the plain delta if never tuned)
     fixed_kw = dict(tuned_b) if tuned_b else dict(delta=args.delta)
-    bopts = Options(delta=args.delta, sched=1, **launch_b) if use_learned else Options(**fixed_kw)
+    bopts = Options(delta=args.delta, sched=1, graph_launch=True, **launch_b) if use_learned else \
+        Options(graph_launch=True, **fixed_kw)
     popts = dict(delta=args.delta, **launch_p)
     log(f"batched options: {bopts}; plain options: {popts}")
 
@@ -122,17 +130,41 @@ def main() -> int:
     log(f"batched: {ms_step:.3f} ms/step wall, kernel {kernel_ms:.3f} ms, {gteps:.2f} GTEPS")
 
     # ---- the autotuned fixed-schedule batched run, for comparison -----------
+    # Both schedules are timed over the same number of launches, alternating,
+    # so that drift affects them alike; median and spread of the kernel times.
     fixed = None
     if use_learned:
-        fo = Options(**fixed_kw)
+        fo = Options(graph_launch=True, **fixed_kw)
         sssp_multi_into(dev, srcs, fo, D)
         assert np.array_equal(D.to_numpy(np.int32, (V, args.nsrc)), Dh), "fixed schedule changed distances"
-        t3 = time.perf_counter()
-        fk = [sssp_multi_into(dev, srcs, fo, D)["ms_kernel"] for _ in range(args.steps)]
-        fwall = time.perf_counter() - t3
-        fixed = {"gteps": edges_per_step * args.steps / fwall / 1e9, "ms_per_step": fwall * 1000.0 / args.steps,
-                 "kernel_ms": float(np.mean(fk)), "options": fixed_kw}
-        log(f"batched fixed {fixed_kw}: {fixed['ms_per_step']:.3f} ms/step, {fixed['gteps']:.2f} GTEPS")
+        n = max(5, args.steps)
+        lk, fk = [], []
+        for _ in range(n):
+            lk.append(sssp_multi_into(dev, srcs, bopts, D)["ms_kernel"])
+            fk.append(sssp_multi_into(dev, srcs, fo, D)["ms_kernel"])
+        lk, fk = np.array(lk), np.array(fk)
+        fixed = {"options": fixed_kw, "launches": n,
+                 "learned_kernel_ms_median": round(float(np.median(lk)), 3),
+                 "learned_kernel_ms_min_max": [round(float(lk.min()), 3), round(float(lk.max()), 3)],
+                 "fixed_kernel_ms_median": round(float(np.median(fk)), 3),
+                 "fixed_kernel_ms_min_max": [round(float(fk.min()), 3), round(float(fk.max()), 3)],
+                 "learned_over_fixed": round(float(np.median(fk) / np.median(lk)), 4)}
+        log(f"learned vs fixed {fixed_kw}: median {fixed['learned_kernel_ms_median']} vs "
+            f"{fixed['fixed_kernel_ms_median']} ms over {n} launches each")
+
+    # ---- with allocation: the step after the cached buffers are freed --------
+    from galoisgpu_amd._lib import check as _check, lib as _lib
+    import dataclasses
+    wopts = dataclasses.replace(bopts, graph_launch=False)  # allocation, not graph capture
+    walloc = []
+    for _ in range(3):
+        _check(_lib().lsg_workspace_release(0), "workspace_release")
+        walloc.append(sssp_multi_into(dev, srcs, wopts, D)["ms_total"])
+    with_alloc = {"ms_per_step_median": round(float(np.median(walloc)), 3),
+                  "gteps": round(edges_per_step / (float(np.median(walloc)) / 1000.0) / 1e9, 3),
+                  "note": "host time of one batched call made right after lsg_workspace_release, so that it "
+                          "allocates its queues, masks and distance matrix as the reference's gg_main does"}
+    log(f"with allocation: {with_alloc['ms_per_step_median']} ms/step")
 
     # ---- plain kernel on the same GPU and sources ---------------------------
     plain = None
@@ -173,8 +205,9 @@ def main() -> int:
         assert all(ok for ok, _ in res), "CPU restatement disagrees with the GPU"
         ce = sum(e for _, e in res)
         cpu = {"value": ce / tcpu / 1e9, "unit": "GTEPS", "cores": threads, "kind": "port",
-               "sample": f"{len(sample)} of the {args.nsrc} sources (those with E_r>0), one per thread, serial "
-                         f"near/far restatement of apps/sssp (oracle/oracle.c), {tcpu:.1f}s wall"}
+               "sample": f"{len(sample)} of the {args.nsrc} sources (those with E_r>0), one per thread on {threads} "
+                         f"of the host's {os.cpu_count()} logical cores, serial near/far restatement of apps/sssp "
+                         f"(oracle/oracle.c), {tcpu:.1f}s wall"}
         log(f"cpu: {cpu['value']:.4f} GTEPS on {len(sample)} sources with {threads} threads in {tcpu:.1f}s")
 
     traffic = lsched.profiled_traffic(ROOT, "sssp_multi_rmat22")
@@ -197,7 +230,8 @@ def main() -> int:
                    "scheduler": "learned" if use_learned else "fixed", "batched_options": str(bopts),
                    "plain_options": popts, "parallelism": "single GPU"},
         "plain": plain,
-        "batched_fixed_schedule": fixed,
+        "learned_vs_fixed_schedule": fixed,
+        "with_allocation": with_alloc,
         "speedup_vs_plain": round(gteps / plain["gteps"], 3) if plain else None,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},

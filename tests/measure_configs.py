@@ -245,6 +245,16 @@ def c5(repeats: int) -> dict:
     out = {"config": "C5", "graph": "rmat24 ef16 w[1,255]", "V": V, "E": g.nedges, "sources": 32, "E_r": er,
            "plain": rate(er, vr, t_plain), "batched": dict(rate(er, vr, ms), learned=learned, options=str(bo)),
            "speedup": round(t_plain / ms, 3)}
+    if learned:
+        # the learned schedule against the fixed Bellman-Ford-round schedule, median of >= 5 launches each
+        fixed = Options(delta=1 << 30, far_mode=0, heavy_deg=256, block_size=256)
+        n = max(5, repeats)
+        ml = sorted(sssp_multi_into(dev, srcs, bo, D)["ms_kernel"] for _ in range(n))
+        mf = sorted(sssp_multi_into(dev, srcs, fixed, D)["ms_kernel"] for _ in range(n))
+        assert np.array_equal(D.to_numpy(np.int32, (V, 32)), Dh), "C5 fixed schedule differs"
+        out["learned_vs_fixed"] = {"launches": n, "learned_ms_median": round(ml[n // 2], 3),
+                                   "fixed_ms_median": round(mf[n // 2], 3), "fixed": str(fixed),
+                                   "learned_over_fixed": round(mf[n // 2] / ml[n // 2], 3)}
     nz = [int(s) for s in srcs if g.row_start[s + 1] > g.row_start[s]][:1]
     out["cpu"] = cpu_nearfar(g, nz, 32)
     D.close()
