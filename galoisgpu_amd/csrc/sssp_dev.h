@@ -158,7 +158,10 @@ struct SsspRelax {
     // round trip fewer, for rounds too small for the extra atomics to matter.
     bool imp = false;
     const bool near = (int64_t)nd <= s.curdelta();
-    uint32_t *stamps = near ? a.near_stamp : a.far_stamp;
+    // select between the pointers as values (selecting between the fields by
+    // a per-lane condition loads the chosen pointer from the argument block)
+    uint32_t *const near_st = a.near_stamp, *const far_st = a.far_stamp;
+    uint32_t *stamps = near ? near_st : far_st;
     const uint32_t stamp = near ? s.round + 1 : s.phase + 1;
     uint32_t seen = stamp;
     if (active) {
@@ -210,6 +213,93 @@ __device__ __forceinline__ void relax_edge(const KArgs &a, const Sched &s, uint3
   }
   if (A == Algo::SSSP) SsspRelax::edge<BS>(a, s, v, nd, active, ps, precheck);
   else BfsVisit::edge<BS>(a, s, v, nd, active, ps, precheck);
+}
+
+// Edges a thread relaxes together.  Each step of one edge's relaxation (edge
+// loads; pre-check loads; atomicMin; stamp exchange) waits for the previous
+// one, so one edge at a time costs about four dependent memory round trips
+// per edge per thread; with kInFlight independent edges each step is issued
+// for all of them before any result is used, and the round trips overlap.
+constexpr int kInFlight = 4;
+
+// The same tests, atomics and pushes as SsspRelax / BfsVisit for the U edges
+// e[k] (leaving vertices at distance du[k]) of the lanes with act[k].  The
+// loads are issued by every lane (inactive lanes read edge 0 and the vertex it
+// leads to, which exist whenever the caller has an edge to relax): loads under
+// per-lane branches make the compiler wait for the whole memory queue at each
+// branch, which would serialise the edges again.  Callers pass e[k] = 0 for
+// inactive lanes.
+template <Algo A, int BS, int U>
+__device__ __forceinline__ void relax_edges(const KArgs &a, const Sched &s, const uint32_t (&e)[U],
+                                            const uint32_t (&du)[U], const bool (&act)[U], PushStage<BS> &ps,
+                                            bool precheck) {
+  uint32_t v[U], nd[U];
+#pragma unroll
+  for (int k = 0; k < U; ++k) {
+    v[k] = glb(a.edge_dst)[e[k]];
+    nd[k] = A == Algo::SSSP ? du[k] + glb(a.edge_data)[e[k]] : du[k] + 1u;
+  }
+  if (A == Algo::SSSP) {
+    const int64_t cd = s.curdelta();
+    // the two stamp arrays as values: selecting between the fields by a
+    // per-lane condition would load the chosen pointer from the argument block
+    uint32_t *const near_st = a.near_stamp, *const far_st = a.far_stamp;
+    uint32_t dv[U], seen[U], stamp[U];
+    bool near[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      near[k] = (int64_t)nd[k] <= cd;
+      stamp[k] = near[k] ? s.round + 1 : s.phase + 1;
+      dv[k] = precheck ? glb(a.dist)[v[k]] : kInf;
+      seen[k] = glb(near[k] ? near_st : far_st)[v[k]];
+    }
+    // old = 0 where no atomic is issued: nd < 0 never holds
+    uint32_t old[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k)
+      old[k] = act[k] && (!precheck || nd[k] < dv[k]) ? atomic_min(a.dist + v[k], nd[k]) : 0u;
+    // Results are taken only here (left to itself the compiler tests each
+    // result inside its atomic's branch, waiting for the whole memory queue).
+#pragma unroll
+    for (int k = 0; k < U; ++k) asm volatile("" : "+v"(old[k]));
+    uint32_t got[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const bool want = nd[k] < old[k] && seen[k] != stamp[k];
+      got[k] = want ? atomic_exch((near[k] ? near_st : far_st) + v[k], stamp[k]) : stamp[k];
+    }
+#pragma unroll
+    for (int k = 0; k < U; ++k) asm volatile("" : "+v"(got[k]));
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const bool imp = nd[k] < old[k];
+      const bool push = imp && got[k] != stamp[k];
+      if (s.aux[1] == LSG_FAR_BUCKET) note_far_min<BS>(ps, imp && !near[k], nd[k]);
+      stage_push<BS>(a, s, ps, 0, push && near[k], v[k]);
+      stage_push<BS>(a, s, ps, 1, push && !near[k], v[k]);
+    }
+  } else {
+    const uint32_t stamp = s.round + 1;
+    uint32_t dv[U], old[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) dv[k] = precheck ? glb(a.dist)[v[k]] : kInf;
+#pragma unroll
+    for (int k = 0; k < U; ++k)
+      old[k] = act[k] && (!precheck || nd[k] < dv[k]) ? atomic_min(a.dist + v[k], nd[k]) : 0u;
+#pragma unroll
+    for (int k = 0; k < U; ++k) asm volatile("" : "+v"(old[k]));
+    uint32_t seen[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) seen[k] = nd[k] < old[k] ? glb(a.near_stamp)[v[k]] : stamp;
+    uint32_t got[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k)
+      got[k] = nd[k] < old[k] && seen[k] != stamp ? atomic_exch(a.near_stamp + v[k], stamp) : stamp;
+#pragma unroll
+    for (int k = 0; k < U; ++k) asm volatile("" : "+v"(got[k]));
+#pragma unroll
+    for (int k = 0; k < U; ++k) stage_push<BS>(a, s, ps, 0, nd[k] < old[k] && got[k] != stamp, v[k]);
+  }
 }
 
 // ---- block-level helpers ---------------------------------------------------
@@ -326,21 +416,27 @@ __device__ void process_tile(const KArgs &a, const Sched &s, uint32_t t, uint32_
     sh.start[threadIdx.x] = rs;
     sh.du[threadIdx.x] = du;
     __syncthreads();
-    for (uint32_t base = 0; base < total; base += BS) {
-      uint32_t e = base + threadIdx.x;
-      bool act = e < total;
-      uint32_t ee = 0, d0 = 0;
-      if (act) {
-        uint32_t lo = 0, hi = BS;
-        while (hi - lo > 1) {
-          uint32_t mid = (lo + hi) >> 1;
-          if (sh.offs[mid] <= e) lo = mid;
-          else hi = mid;
+    for (uint32_t base = 0; base < total; base += BS * kInFlight) {
+      uint32_t ee[kInFlight], d0[kInFlight];
+      bool act[kInFlight];
+#pragma unroll
+      for (int k = 0; k < kInFlight; ++k) {
+        const uint32_t e = base + k * BS + threadIdx.x;
+        act[k] = e < total;
+        ee[k] = 0;
+        d0[k] = 0;
+        if (act[k]) {
+          uint32_t lo = 0, hi = BS;
+          while (hi - lo > 1) {
+            uint32_t mid = (lo + hi) >> 1;
+            if (sh.offs[mid] <= e) lo = mid;
+            else hi = mid;
+          }
+          ee[k] = sh.start[lo] + (e - sh.offs[lo]);
+          d0[k] = sh.du[lo];
         }
-        ee = sh.start[lo] + (e - sh.offs[lo]);
-        d0 = sh.du[lo];
       }
-      relax_edge<A, BS>(a, s, ee, d0, act, sh.ps, precheck);
+      relax_edges<A, BS, kInFlight>(a, s, ee, d0, act, sh.ps, precheck);
     }
     __syncthreads();
   } else {
@@ -378,9 +474,16 @@ __device__ void process_heavy(const KArgs &a, const Sched &s, uint32_t c, TileSh
   uint32_t lo = rs + k * a.heavy_deg;
   uint32_t hi = min(re, lo + a.heavy_deg);
   if ((a.sched || a.feat) && threadIdx.x == 0) atomicAdd(&sh.edges, (unsigned long long)(hi - lo));
-  for (uint32_t b = lo; b < hi; b += BS) {
-    uint32_t e = b + threadIdx.x;
-    relax_edge<A, BS>(a, s, e, du, e < hi, sh.ps, true);
+  for (uint32_t b = lo; b < hi; b += BS * kInFlight) {
+    uint32_t ee[kInFlight], d0[kInFlight];
+    bool act[kInFlight];
+#pragma unroll
+    for (int k = 0; k < kInFlight; ++k) {
+      act[k] = k * BS + threadIdx.x < hi - b;  // b < hi: no wrap-around near 2^32 edges
+      ee[k] = act[k] ? b + k * BS + threadIdx.x : 0u;
+      d0[k] = du;
+    }
+    relax_edges<A, BS, kInFlight>(a, s, ee, d0, act, sh.ps, true);
   }
 }
 
@@ -567,8 +670,9 @@ __device__ void local_rounds(const KArgs &a, TileShared<BS> &sh, Sched &ls, uint
   }
 }
 
+// At least 4 waves per SIMD (at most 128 VGPRs): two blocks of 512 per CU.
 template <Algo A, int BS>
-__global__ __launch_bounds__(BS) void persistent_kernel(KArgs a) {
+__global__ __launch_bounds__(BS, 4) void persistent_kernel(KArgs a) {
   __shared__ TileShared<BS> sh;
   __shared__ Sched ss;
   __shared__ Sched lsched;  // the leader step's schedule, reused by local rounds
