@@ -618,6 +618,7 @@ void multi_kernel(MArgs a, MultiCounters *tot) {
   __shared__ unsigned long long blk[3];
   __shared__ DistSlots dls[BS / kWave];
   __shared__ uint32_t pushbuf[BS / kWave][2 * kPushBuf];
+  __shared__ uint32_t drain_n[BS / kWave][2], drain_base[2];
   Ctrl *c = a.ctrl;
   const uint32_t nb = gridDim.x;
   if (!residency_handshake(c, nb, a.residency_limit)) return;
@@ -677,20 +678,50 @@ void multi_kernel(MArgs a, MultiCounters *tot) {
       }
     }
     flush_pending(a, s, ls, p);
-    push_flush(a, s, ls, 0);
-    push_flush(a, s, ls, 1);
     // this wavefront's smallest far push per source -> phase minimum
     if (__ballot(ls.fmin != kInf) && ls.fmin != kInf) atomic_min(a.fmin + (s.phase & 1) * 64 + lane_id(), ls.fmin);
-    if (lane_id() == 0 && (mc.lane_items | mc.edges)) {
-      atomicAdd(&blk[0], (unsigned long long)mc.lane_items);
-      atomicAdd(&blk[1], (unsigned long long)mc.edges);
-      atomicAdd(&blk[2], (unsigned long long)mc.improved);
+    const uint32_t wv = threadIdx.x / kWave;
+    if (lane_id() == 0) {
+      drain_n[wv][0] = ls.pbn[0];
+      drain_n[wv][1] = ls.pbn[1];
+      if (mc.lane_items | mc.edges) {
+        atomicAdd(&blk[0], (unsigned long long)mc.lane_items);
+        atomicAdd(&blk[1], (unsigned long long)mc.edges);
+        atomicAdd(&blk[2], (unsigned long long)mc.improved);
+      }
     }
     __syncthreads();
+    // The block's staged pushes get their places with one queue-tail atomic
+    // per pile (one per wavefront, 4096 on one tail, serialise for ~50 us at
+    // the memory-side atomic unit every round).
+    if (threadIdx.x < 2) {
+      const uint32_t which = threadIdx.x;
+      uint32_t tot_n = 0;
+      for (uint32_t w = 0; w < BS / kWave; ++w) {
+        const uint32_t n = drain_n[w][which];
+        drain_n[w][which] = tot_n;  // becomes the wavefront's offset
+        tot_n += n;
+      }
+      drain_base[which] = tot_n ? atomic_add(&c->qcount[which ? s.far_q : s.near_q], tot_n) : 0u;
+    }
     if (threadIdx.x == 0 && (blk[0] | blk[1])) {
       __hip_atomic_fetch_add(&tot->lane_items, (uint64_t)blk[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_fetch_add(&tot->edges, (uint64_t)blk[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_fetch_add(&tot->improved, (uint64_t)blk[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int which = 0; which < 2; ++which) {
+      const uint32_t n = ls.pbn[which];
+      if (n == 0) continue;
+      const uint32_t qi = which ? s.far_q : s.near_q;
+      const uint32_t base = drain_base[which] + drain_n[wv][which];
+      const uint32_t *buf = ls.pb + which * kPushBuf;
+      for (uint32_t i = lane_id(); i < n; i += kWave) {
+        const uint32_t pos = base + i;
+        if (pos < a.qcap) st_coh(a.q[qi] + pos, buf[i]);
+        else st_coh(&a.ctrl->abort, kAbortCapacity);
+      }
     }
     grid_barrier_block(c, nb, a.spin_limit, [&] { multi_leader(a, tot); });
   }

@@ -77,6 +77,9 @@ struct PushStage {
   // tails are kept here and stored once at the end of the round.
   uint32_t local;
   uint32_t tail[2];
+  // End-of-round drain: each wavefront's place in the block's reservation.
+  uint32_t off[BS / kWave][2];
+  uint32_t base[2];
 };
 
 // Writes this wavefront's staged items of pile `which` (0 near, 1 far) to the queue.
@@ -114,11 +117,53 @@ __device__ __forceinline__ void stage_push(const KArgs &a, const Sched &s, PushS
   ps.n[w][which] = n + c;
 }
 
-// End of a round: every wavefront writes out what it staged.
+// End of a round, called by every thread of the block: every wavefront writes
+// out what it staged, in places reserved for the whole block with one
+// queue-tail atomic per pile.  (One atomic per wavefront, ~2048 of them on
+// one tail when every wavefront of the grid pushes, serialised for ~25 us at
+// the memory-side atomic unit: most of a round's time on the 2048^2 road graph.)
 template <int BS>
 __device__ __forceinline__ void stage_drain(const KArgs &a, const Sched &s, PushStage<BS> &ps) {
-  stage_flush<BS>(a, s, ps, 0);
-  stage_flush<BS>(a, s, ps, 1);
+  constexpr int NW = BS / kWave;
+  __syncthreads();  // every wavefront's staged count is final
+  if (threadIdx.x < 2) {
+    const int which = (int)threadIdx.x;  // lane 0: near pile, lane 1: far pile
+    uint32_t tot = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      ps.off[w][which] = tot;
+      tot += ps.n[w][which];
+    }
+    uint32_t base = 0;
+    if (tot) {
+      if (ps.local) {
+        // only this block pushes during a block-local round, and every
+        // wavefront is past its last flush
+        base = ps.tail[which];
+        ps.tail[which] = base + tot;
+      } else {
+        base = atomic_add(&a.ctrl->qcount[which ? s.far_q : s.near_q], tot);
+      }
+    }
+    ps.base[which] = base;
+  }
+  __syncthreads();
+  const uint32_t w = threadIdx.x / kWave;
+#pragma unroll
+  for (int which = 0; which < 2; ++which) {
+    const uint32_t n = __builtin_amdgcn_readfirstlane(ps.n[w][which]);
+    if (n == 0) continue;
+    const uint32_t qi = which ? s.far_q : s.near_q;
+    const uint32_t base = ps.base[which] + ps.off[w][which];
+    for (uint32_t i = lane_id(); i < n; i += kWave) {
+      const uint32_t pos = base + i;
+      if (pos < a.qcap) st_coh(a.q[qi] + pos, ps.item[w][which][i]);
+      else st_coh(&a.ctrl->abort, kAbortCapacity);
+    }
+  }
+  // every lane has read the wavefront's count before lane 0 clears it
+  __builtin_amdgcn_wave_barrier();
+  if (lane_id() == 0) ps.n[w][0] = ps.n[w][1] = 0;
 }
 
 // Bucket far mode: folds the smallest `x` of the lanes with `pred` into the
