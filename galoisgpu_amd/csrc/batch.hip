@@ -500,11 +500,14 @@ __device__ __forceinline__ void multi_leader(const MArgs &a, MultiCounters *tot)
   Ctrl *c = a.ctrl;
   const uint32_t t = threadIdx.x;
   if (t == 0) {
-    // every control word of the step in one batch of loads
-    sched_load(s, &c->s);
+    // every control word of the step in one batch of loads (the schedule to
+    // registers first, so that the other loads need not wait for LDS stores)
+    Sched sr;
+    sched_load(sr, &c->s);
     CtrlCounts k;
     counts_load(k, c);
     const uint64_t tot_lane_items = ld_coh64(&tot->lane_items), tot_edges = ld_coh64(&tot->edges);
+    s = sr;
     // every wavefront has left the round's item loops: restart the tickets
     st_coh(&c->ticket, 0u);
     st_coh(&c->ticket2, 0u);

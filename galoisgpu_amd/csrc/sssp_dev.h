@@ -553,11 +553,15 @@ __device__ void leader_step(const KArgs &a, Sched &s, uint32_t &abort_seen, uint
   if (threadIdx.x == 0) {
     decide = 0;
     // Every control word of the step is loaded in one batch: the step costs
-    // one round trip to the coherence point instead of one per dependent load.
-    sched_load(s, &c->s);
+    // one round trip to the coherence point instead of one per dependent load
+    // (the schedule goes to registers first: loaded straight into LDS, the
+    // counters' loads would wait for its stores).
+    Sched sr;
+    sched_load(sr, &c->s);
     CtrlCounts k;
     counts_load(k, c);
     abort_seen = ld_coh(&c->abort);
+    s = sr;
     far_min = k.far_min;
     const uint32_t nnear = k.q(s.near_q), nfar = k.q(s.far_q), nheavy = k.h(1u - s.heavy_in);
     far_tail = 0;
