@@ -455,12 +455,34 @@ __device__ void process_tile(const KArgs &a, const Sched &s, uint32_t t, uint32_
     // Tile-flattened expansion: prefix sum of degrees, then every lane takes
     // one edge of the flattened tile (the reference's fg scheduler,
     // apps/sssp/kernel.cu:266-302, at block granularity).
+    // A tile of at most 64 items (all in wavefront 0: most rounds of a
+    // high-diameter graph) is scanned by that wavefront alone, with one
+    // block barrier instead of four.
+    const uint32_t nitems = min(ti, s.in_count - t * ti);
+    const uint32_t span = nitems <= (uint32_t)kWave ? (uint32_t)kWave : (uint32_t)BS;
     uint32_t total;
-    uint32_t off = block_exclusive_sum<BS>(deg, &total, sh.wsum);
-    sh.offs[threadIdx.x] = off;
-    sh.start[threadIdx.x] = rs;
-    sh.du[threadIdx.x] = du;
-    __syncthreads();
+    if (span == (uint32_t)kWave) {
+      if (threadIdx.x < (uint32_t)kWave) {
+        uint32_t incl = deg;
+#pragma unroll
+        for (int d = 1; d < kWave; d <<= 1) {
+          const uint32_t y = __shfl_up(incl, d);
+          if (lane_id() >= (uint32_t)d) incl += y;
+        }
+        sh.offs[threadIdx.x] = incl - deg;
+        sh.start[threadIdx.x] = rs;
+        sh.du[threadIdx.x] = du;
+        if (threadIdx.x == kWave - 1) sh.wsum[0] = incl;
+      }
+      __syncthreads();
+      total = sh.wsum[0];
+    } else {
+      const uint32_t off = block_exclusive_sum<BS>(deg, &total, sh.wsum);
+      sh.offs[threadIdx.x] = off;
+      sh.start[threadIdx.x] = rs;
+      sh.du[threadIdx.x] = du;
+      __syncthreads();
+    }
     for (uint32_t base = 0; base < total; base += BS * kInFlight) {
       uint32_t ee[kInFlight], d0[kInFlight];
       bool act[kInFlight];
@@ -471,7 +493,7 @@ __device__ void process_tile(const KArgs &a, const Sched &s, uint32_t t, uint32_
         ee[k] = 0;
         d0[k] = 0;
         if (act[k]) {
-          uint32_t lo = 0, hi = BS;
+          uint32_t lo = 0, hi = span;
           while (hi - lo > 1) {
             uint32_t mid = (lo + hi) >> 1;
             if (sh.offs[mid] <= e) lo = mid;
