@@ -211,10 +211,13 @@ struct SsspRelax {
     uint32_t seen = stamp;
     if (active) {
       const uint32_t dv = precheck ? glb(a.dist)[v] : kInf;
-      // the stamp is loaded beside d[v]: one round trip fewer when the edge
-      // improves (C1 12.3 -> 11.9 ms, C3 12.0 -> 11.9 ms, RMAT unchanged)
-      seen = glb(stamps)[v];
-      if (!precheck || nd < dv) imp = nd < atomic_min(a.dist + v, nd);
+      if (!precheck || nd < dv) {
+        // the stamp is loaded with the atomicMin, and only for candidates:
+        // most edges of a low-diameter graph fail the pre-check, and each
+        // stamp load is a random line fetch
+        seen = glb(stamps)[v];
+        imp = nd < atomic_min(a.dist + v, nd);
+      }
     }
     // Push once per round (near) or phase (far): the exchange decides between
     // racing improvers, the cached pre-check skips it when already stamped.
@@ -296,13 +299,19 @@ __device__ __forceinline__ void relax_edges(const KArgs &a, const Sched &s, cons
       near[k] = (int64_t)nd[k] <= cd;
       stamp[k] = near[k] ? s.round + 1 : s.phase + 1;
       dv[k] = precheck ? glb(a.dist)[v[k]] : kInf;
-      seen[k] = glb(near[k] ? near_st : far_st)[v[k]];
     }
-    // old = 0 where no atomic is issued: nd < 0 never holds
+    // The stamps are loaded with the atomicMins, and only for candidates
+    // (other lanes read vertex 0's, one line for the whole wavefront): most
+    // edges of a low-diameter graph fail the pre-check, and each stamp load
+    // is a random line fetch.  old = 0 where no atomic is issued: nd < 0
+    // never holds.
     uint32_t old[U];
 #pragma unroll
-    for (int k = 0; k < U; ++k)
-      old[k] = act[k] && (!precheck || nd[k] < dv[k]) ? atomic_min(a.dist + v[k], nd[k]) : 0u;
+    for (int k = 0; k < U; ++k) {
+      const bool cand = act[k] && (!precheck || nd[k] < dv[k]);
+      seen[k] = glb(near[k] ? near_st : far_st)[cand ? v[k] : 0u];
+      old[k] = cand ? atomic_min(a.dist + v[k], nd[k]) : 0u;
+    }
     // Results are taken only here (left to itself the compiler tests each
     // result inside its atomic's branch, waiting for the whole memory queue).
 #pragma unroll
@@ -325,17 +334,18 @@ __device__ __forceinline__ void relax_edges(const KArgs &a, const Sched &s, cons
     }
   } else {
     const uint32_t stamp = s.round + 1;
-    uint32_t dv[U], old[U];
+    uint32_t dv[U], old[U], seen[U];
 #pragma unroll
     for (int k = 0; k < U; ++k) dv[k] = precheck ? glb(a.dist)[v[k]] : kInf;
+    // stamps loaded with the atomicMins, for candidates only (as for SSSP)
 #pragma unroll
-    for (int k = 0; k < U; ++k)
-      old[k] = act[k] && (!precheck || nd[k] < dv[k]) ? atomic_min(a.dist + v[k], nd[k]) : 0u;
+    for (int k = 0; k < U; ++k) {
+      const bool cand = act[k] && (!precheck || nd[k] < dv[k]);
+      seen[k] = glb(a.near_stamp)[cand ? v[k] : 0u];
+      old[k] = cand ? atomic_min(a.dist + v[k], nd[k]) : 0u;
+    }
 #pragma unroll
     for (int k = 0; k < U; ++k) asm volatile("" : "+v"(old[k]));
-    uint32_t seen[U];
-#pragma unroll
-    for (int k = 0; k < U; ++k) seen[k] = nd[k] < old[k] ? glb(a.near_stamp)[v[k]] : stamp;
     uint32_t got[U];
 #pragma unroll
     for (int k = 0; k < U; ++k)
