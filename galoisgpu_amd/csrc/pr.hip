@@ -16,8 +16,11 @@
 //
 // Layout: rank and contribution arrays f32[V], two of each (current and
 // next); the reverse CSR from lsg_graph_transpose; a list of the vertices
-// with in-degree >= kWaveDeg, which a whole wavefront sums (the others take
-// one lane each).
+// with in-degree >= kWaveDeg (the others take one lane each), whose in-edges
+// are cut into chunks of kChunk: one wavefront sums a chunk into a partial
+// (part[]), then one lane per vertex adds its partials in chunk order.  (One
+// wavefront per vertex left the whole sweep waiting on the largest in-degree,
+// ~97 K edges on RMAT-22: ~0.4 ms of a 0.72 ms iteration.)
 //
 // The convergence test stays on the device: the host enqueues iterations in
 // batches; after each iteration a one-thread step kernel counts it and sets a
@@ -50,13 +53,18 @@ namespace lsg {
 
 constexpr uint32_t kWaveDeg = 32;  // in-degree from which a wavefront sums a vertex
 
+constexpr uint32_t kChunk = 4096;  // in-edges per wavefront-sized partial sum
+
 struct PrArgs {
   const uint32_t *row_start;         // forward CSR (out-degrees)
   const uint32_t *in_row, *in_src;   // reverse CSR
   const float *c_cur;                // contributions of the current ranks
   float *p_next, *c_next;
   const uint32_t *heavy;             // vertices with in-degree >= kWaveDeg
-  uint32_t nheavy, nnodes;
+  const uint32_t *hbase;             // heavy vertex i's first partial; its partials are consecutive
+  const uint32_t *chunk;             // partial k belongs to heavy vertex chunk[k] (chunk number k - hbase[.])
+  float *part;                       // partial sums, kChunk in-edges each
+  uint32_t nheavy, nchunks, nnodes;
   float alpha;
   uint32_t *maxdiff;                 // bits of the largest |p_next - p_cur| / p_next (non-negative floats order as u32)
   const float *p_cur;
@@ -67,7 +75,7 @@ constexpr uint32_t kIterBatch = 8;  // iterations enqueued per host check
 
 // control words; the largest change is kept per iteration parity, so the step
 // kernel can clear the next iteration's word while this one's is read
-constexpr int kCtlHeavy = 0, kCtlMax = 1, kCtlDone = 3, kCtlIters = 4;
+constexpr int kCtlHeavy = 0, kCtlMax = 1, kCtlDone = 3, kCtlIters = 4, kCtlChunks = 5;
 
 // Whether an earlier iteration converged (written by pr_step_kernel in an
 // earlier launch, so a plain load sees it).
@@ -93,6 +101,37 @@ __global__ void pr_init_kernel(const uint32_t *row_start, const uint32_t *in_row
       if (lane_id() == leader) base = atomic_add(nheavy, (uint32_t)__builtin_popcountll(m));
       base = readlane(base, leader);
       if (h) heavy[base + lanes_below(m)] = (uint32_t)v;
+    }
+  }
+}
+
+// After pr_init_kernel: heavy vertex i gets ceil(indeg / kChunk) consecutive
+// partials starting at hbase[i] (where they sit depends on the order of the
+// atomics; the order in which a vertex's partials are added never does).
+__global__ void pr_chunks_kernel(const uint32_t *in_row, const uint32_t *heavy, uint32_t nheavy, uint32_t *hbase,
+                                 uint32_t *chunk, uint32_t *nchunks) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  const uint32_t lane = lane_id();
+  // wavefront-uniform loop: one reservation atomic per wavefront (one per
+  // vertex, ~280 K on one word for RMAT-22, would serialise for milliseconds)
+  for (size_t i0 = blockIdx.x * (size_t)blockDim.x + (threadIdx.x & ~(uint32_t)(kWave - 1)); i0 < nheavy; i0 += stride) {
+    const size_t i = i0 + lane;
+    const bool ok = i < nheavy;
+    const uint32_t v = ok ? heavy[i] : 0u;
+    const uint32_t n = ok ? (in_row[v + 1] - in_row[v] + kChunk - 1) / kChunk : 0u;
+    uint32_t incl = n;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+      const uint32_t y = __shfl_up(incl, d);
+      if (lane >= (uint32_t)d) incl += y;
+    }
+    const uint32_t total = readlane(incl, kWave - 1);
+    uint32_t base = 0;
+    if (lane == 0 && total) base = atomic_add(nchunks, total);
+    base = readlane(base, 0) + incl - n;
+    if (ok) {
+      hbase[i] = base;
+      for (uint32_t k = 0; k < n; ++k) chunk[base + k] = (uint32_t)i;
     }
   }
 }
@@ -146,31 +185,48 @@ __global__ void pr_light_kernel(PrArgs a) {
   pr_reduce_max(a, dmax);
 }
 
-// One wavefront per vertex of in-degree >= kWaveDeg: 64 in-edges per step,
-// then a butterfly sum (the same order every iteration).
-__global__ void pr_heavy_kernel(PrArgs a) {
+// One wavefront per partial: kChunk in-edges of a heavy vertex, 64 per step
+// (four steps' loads in flight), then a butterfly sum -- the same order every
+// iteration.
+__global__ void pr_chunk_kernel(PrArgs a) {
   if (pr_done(a)) return;
-  float dmax = 0.0f;
   const uint32_t lane = lane_id();
   const size_t waves = (size_t)gridDim.x * blockDim.x / kWave;
-  for (size_t i = (blockIdx.x * (size_t)blockDim.x + threadIdx.x) / kWave; i < a.nheavy; i += waves) {
+  for (size_t k = (blockIdx.x * (size_t)blockDim.x + threadIdx.x) / kWave; k < a.nchunks; k += waves) {
+    const uint32_t i = a.chunk[k];
     const uint32_t v = a.heavy[i];
-    const uint32_t e0 = a.in_row[v], e1 = a.in_row[v + 1];
+    const uint32_t e0 = a.in_row[v] + (uint32_t)(k - a.hbase[i]) * kChunk;
+    const uint32_t e1 = min(a.in_row[v + 1], e0 + kChunk);
     // lane l sums in-edges e0 + l, e0 + l + 64, ... in that order, four per step
     float sum = 0.0f;
     for (uint32_t e = e0 + lane; e < e1; e += 4 * kWave) {
       uint32_t u[4];
       float c[4];
 #pragma unroll
-      for (int k = 0; k < 4; ++k) u[k] = e + k * kWave < e1 ? a.in_src[e + k * kWave] : 0u;
+      for (int q = 0; q < 4; ++q) u[q] = e + q * kWave < e1 ? a.in_src[e + q * kWave] : 0u;
 #pragma unroll
-      for (int k = 0; k < 4; ++k) c[k] = e + k * kWave < e1 ? a.c_cur[u[k]] : 0.0f;
+      for (int q = 0; q < 4; ++q) c[q] = e + q * kWave < e1 ? a.c_cur[u[q]] : 0.0f;
 #pragma unroll
-      for (int k = 0; k < 4; ++k) sum += c[k];
+      for (int q = 0; q < 4; ++q) sum += c[q];
     }
 #pragma unroll
     for (int d = kWave / 2; d >= 1; d >>= 1) sum += __shfl_xor(sum, d);
-    if (lane == 0) pr_store(a, v, sum, dmax);
+    if (lane == 0) a.part[k] = sum;
+  }
+}
+
+// One lane per heavy vertex: its partials added in chunk order.
+__global__ void pr_finish_kernel(PrArgs a) {
+  if (pr_done(a)) return;
+  float dmax = 0.0f;
+  const size_t nt = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < a.nheavy; i += nt) {
+    const uint32_t v = a.heavy[i];
+    const uint32_t n = (a.in_row[v + 1] - a.in_row[v] + kChunk - 1) / kChunk;
+    const float *pp = a.part + a.hbase[i];
+    float sum = 0.0f;
+    for (uint32_t k = 0; k < n; ++k) sum += pp[k];
+    pr_store(a, v, sum, dmax);
   }
   pr_reduce_max(a, dmax);
 }
@@ -213,15 +269,23 @@ int lsg_pagerank(const lsg_graph *g, const lsg_graph *gt, float alpha, float eps
   HIPCHK(hipSetDevice(w->device));
   const uint32_t V = g->nnodes;
   void *buf, *cbuf;
-  // p[2], c[2] (f32[V] each) and the heavy list (u32[V])
-  if ((rc = ws_buffer(w, kBufQueues, 5ull * V * 4, &buf)) || (rc = ws_buffer(w, kBufSeeds, 2048, &cbuf))) return rc;
+  // p[2], c[2] (f32[V] each), the heavy list and its partials' bases (u32[V]
+  // each), and per partial its heavy vertex (u32) and sum (f32): at most
+  // V + E / kChunk partials
+  const uint64_t maxchunks = (uint64_t)V + g->nedges / kChunk + 1;
+  if ((rc = ws_buffer(w, kBufQueues, (6ull * V + 2 * maxchunks) * 4, &buf)) ||
+      (rc = ws_buffer(w, kBufSeeds, 2048, &cbuf)))
+    return rc;
   float *p[2] = {(float *)buf, (float *)buf + V};
   float *c[2] = {(float *)buf + 2ull * V, (float *)buf + 3ull * V};
   uint32_t *heavy = (uint32_t *)buf + 4ull * V;
-  uint32_t *ctl = (uint32_t *)cbuf;  // heavy count, max change (x2), done, iterations
+  uint32_t *hbase = (uint32_t *)buf + 5ull * V;
+  uint32_t *chunk = (uint32_t *)buf + 6ull * V;
+  float *part = (float *)buf + 6ull * V + maxchunks;
+  uint32_t *ctl = (uint32_t *)cbuf;  // heavy count, max change (x2), done, iterations, partial count
   const uint32_t vgrid = (uint32_t)std::min<uint64_t>(((uint64_t)V + 255) / 256, 16ull * w->cus);
   HIPCHK(hipEventRecord(w->ev0, w->stream));
-  hipLaunchKernelGGL(pr_clear_kernel, dim3(1), dim3(64), 0, w->stream, ctl, 5u);
+  hipLaunchKernelGGL(pr_clear_kernel, dim3(1), dim3(64), 0, w->stream, ctl, 6u);
   hipLaunchKernelGGL(pr_init_kernel, dim3(vgrid), dim3(256), 0, w->stream, g->row_start, gt->row_start, V, alpha,
                      p[0], c[0], heavy, ctl);
   HIPCHK(hipGetLastError());
@@ -229,16 +293,32 @@ int lsg_pagerank(const lsg_graph *g, const lsg_graph *gt, float alpha, float eps
   HIPCHK(hipMemcpyAsync(hc, ctl, 4, hipMemcpyDeviceToHost, w->stream));
   HIPCHK(hipStreamSynchronize(w->stream));
   const uint32_t nheavy = hc[0];
+  const uint32_t fgrid = (uint32_t)std::min<uint64_t>(((uint64_t)nheavy + 255) / 256 + 1, 16ull * w->cus);
+  if (nheavy) {
+    hipLaunchKernelGGL(pr_chunks_kernel, dim3(fgrid), dim3(256), 0, w->stream, gt->row_start, heavy, nheavy, hbase,
+                       chunk, ctl + kCtlChunks);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(hc, ctl + kCtlChunks, 4, hipMemcpyDeviceToHost, w->stream));
+    HIPCHK(hipStreamSynchronize(w->stream));
+  } else {
+    hc[0] = 0;
+  }
+  const uint32_t nchunks = hc[0];
+  if (nchunks > maxchunks) return fail(LSG_ERR_CAPACITY, "pagerank: %u partial sums exceed the buffer", nchunks);
   PrArgs a;
   a.row_start = g->row_start;
   a.in_row = gt->row_start;
   a.in_src = gt->edge_dst;
   a.heavy = heavy;
+  a.hbase = hbase;
+  a.chunk = chunk;
+  a.part = part;
   a.nheavy = nheavy;
+  a.nchunks = nchunks;
   a.nnodes = V;
   a.alpha = alpha;
   a.done = ctl + kCtlDone;
-  const uint32_t hgrid = (uint32_t)std::min<uint64_t>(((uint64_t)nheavy * kWave + 255) / 256 + 1, 16ull * w->cus);
+  const uint32_t cgrid = (uint32_t)std::min<uint64_t>(((uint64_t)nchunks * kWave + 255) / 256 + 1, 16ull * w->cus);
   // Iteration `it` reads p[it & 1], c[it & 1] and writes the other pair.
   for (uint32_t it = 0; it < (uint32_t)max_iterations;) {
     const uint32_t end = std::min<uint32_t>((uint32_t)max_iterations, it + kIterBatch);
@@ -249,7 +329,10 @@ int lsg_pagerank(const lsg_graph *g, const lsg_graph *gt, float alpha, float eps
       a.c_next = c[(it & 1) ^ 1];
       a.maxdiff = ctl + kCtlMax + (it & 1);
       hipLaunchKernelGGL(pr_light_kernel, dim3(vgrid), dim3(256), 0, w->stream, a);
-      if (nheavy) hipLaunchKernelGGL(pr_heavy_kernel, dim3(hgrid), dim3(256), 0, w->stream, a);
+      if (nheavy) {
+        hipLaunchKernelGGL(pr_chunk_kernel, dim3(cgrid), dim3(256), 0, w->stream, a);
+        hipLaunchKernelGGL(pr_finish_kernel, dim3(fgrid), dim3(256), 0, w->stream, a);
+      }
       hipLaunchKernelGGL(pr_step_kernel, dim3(1), dim3(1), 0, w->stream, ctl, it, epsilon);
     }
     HIPCHK(hipGetLastError());
