@@ -18,15 +18,13 @@
 // found once by binary search) followed by pointer jumping to the root for
 // every vertex.
 //
-// The fixpoint test stays on the device: the host enqueues passes in
-// batches, a one-thread step kernel after each pass sets the done flag when
-// the pass hooked nothing, and every kernel of a later pass returns at once
-// when it finds the flag set.  The host reads the flag once per batch
-// instead of once per pass (the reference copies its flag back after every
-// pass, apps/cc/kernel.cu:690-716).  The passes stay separate launches: a
-// launch boundary is what makes one pass's labels visible to every XCD's L2
-// for the next, which a persistent kernel would have to buy with an L2
-// invalidate in every block at every pass.
+// All passes run in one persistent launch on the runtime the traversals use
+// (residency handshake, leader grid barrier): a hook pass, a barrier, a
+// jump pass, and a barrier whose leader ends the loop when the pass hooked
+// nothing (the reference's gg_main_pipe_4 loop, apps/cc/kernel.cu:465-707,
+// which copies its flag back to the host after every pass instead).  Every
+// label access within the launch is a coherent load or an atomic, so blocks
+// on different XCDs see each other's hooks without cache maintenance.
 
 #include <hip/hip_runtime.h>
 
@@ -41,24 +39,14 @@
 namespace lsg {
 
 constexpr uint32_t kHookRun = 16;  // edges per thread in a hook pass
-constexpr uint32_t kPassBatch = 4;  // passes enqueued per host check
 
-// control words
-constexpr int kCtlChanged = 0, kCtlCount = 1, kCtlDone = 2, kCtlPasses = 3;
+// control word of the component count
+constexpr int kCtlCount = 1;
 
 __global__ void cc_init_kernel(uint32_t *label, uint32_t V, uint32_t *ctl) {
   for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < V; v += (size_t)gridDim.x * blockDim.x)
     label[v] = (uint32_t)v;
   if (blockIdx.x == 0 && threadIdx.x < 4) ctl[threadIdx.x] = 0;
-}
-
-// Whether an earlier pass already reached the fixpoint (written by
-// cc_step_kernel in an earlier launch, so a plain load sees it).
-__device__ __forceinline__ bool cc_done(const uint32_t *ctl) {
-  __shared__ uint32_t s_done;
-  if (threadIdx.x == 0) s_done = ctl[kCtlDone];
-  __syncthreads();
-  return s_done != 0;
 }
 
 // Root of v: follows parents (every parent id is <= the child's, so the walk
@@ -75,9 +63,8 @@ __device__ __forceinline__ uint32_t find_root(const uint32_t *label, uint32_t v)
 // Hook pass: for every edge (u, v), if u's and v's roots differ, the larger
 // root is hooked under the smaller with atomicMin; ctl[0] is set if any hook
 // lowered a parent.
-__global__ void cc_hook_kernel(const uint32_t *row_start, const uint32_t *edge_dst, uint32_t V, uint32_t E,
-                               uint32_t *label, uint32_t *ctl) {
-  if (cc_done(ctl)) return;
+__device__ bool cc_hook_pass(const uint32_t *row_start, const uint32_t *edge_dst, uint32_t V, uint32_t E,
+                             uint32_t *label) {
   const size_t nthreads = (size_t)gridDim.x * blockDim.x;
   bool changed = false;
   for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t * kHookRun < E; t += nthreads) {
@@ -102,22 +89,70 @@ __global__ void cc_hook_kernel(const uint32_t *row_start, const uint32_t *edge_d
       ru = find_root(label, u);
     }
   }
-  // one store per block that changed anything: a store per wavefront to the
-  // same word serialises like a same-address atomic
-  __shared__ uint32_t s_changed;
-  if (threadIdx.x == 0) s_changed = 0;
-  __syncthreads();
-  if (__ballot(changed) && lane_id() == 0) s_changed = 1;
-  __syncthreads();
-  if (threadIdx.x == 0 && s_changed) st_coh(ctl + kCtlChanged, 1u);
+  return changed;
 }
 
 // Pointer jumping: every vertex's parent becomes its root.
-__global__ void cc_jump_kernel(uint32_t *label, uint32_t V, const uint32_t *ctl) {
-  if (cc_done(ctl)) return;
+__device__ void cc_jump_pass(uint32_t *label, uint32_t V) {
   for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < V; v += (size_t)gridDim.x * blockDim.x) {
     const uint32_t r = find_root(label, (uint32_t)v);
     if (ld_coh(label + v) != r) atomic_min(label + v, r);
+  }
+}
+
+struct CcArgs {
+  const uint32_t *row_start, *edge_dst;
+  uint32_t *label;
+  Ctrl *ctrl;
+  uint32_t V, E;
+  uint32_t max_passes;
+  uint64_t spin_limit, residency_limit;
+};
+
+// Schedule words: round = passes done, done; aux[p & 1] = "pass p hooked".
+constexpr uint32_t kCcBlock = 256;
+
+__global__ __launch_bounds__(kCcBlock) void cc_persistent_kernel(CcArgs a) {
+  __shared__ uint32_t s_changed, quit;
+  Ctrl *c = a.ctrl;
+  const uint32_t nb = gridDim.x;
+  if (!residency_handshake(c, nb, a.residency_limit)) return;
+  for (uint32_t pass = 0;; ++pass) {
+    const bool changed = cc_hook_pass(a.row_start, a.edge_dst, a.V, a.E, a.label);
+    // one store per block that changed anything (a store per wavefront to
+    // the same word serialises like a same-address atomic)
+    if (threadIdx.x == 0) s_changed = 0;
+    __syncthreads();
+    if (__ballot(changed) && lane_id() == 0) s_changed = 1;
+    __syncthreads();
+    if (threadIdx.x == 0 && s_changed) st_coh(&c->s.aux[pass & 1], 1u);
+    grid_barrier_block(c, nb, a.spin_limit, [] {});
+    cc_jump_pass(a.label, a.V);
+    grid_barrier_block(c, nb, a.spin_limit, [&] {
+      if (threadIdx.x == 0) {
+        const uint32_t hooked = ld_coh(&c->s.aux[pass & 1]);
+        st_coh(&c->s.aux[(pass + 1) & 1], 0u);  // the next pass's flag (nobody writes it before the release)
+        st_coh(&c->s.round, pass + 1);
+        if (!hooked || pass + 1 >= a.max_passes) st_coh(&c->s.done, 1u);
+      }
+    });
+    if (threadIdx.x == 0) quit = ld_coh(&c->s.done) != 0 || ld_coh(&c->abort) != kAbortNone;
+    __syncthreads();
+    if (quit) break;
+  }
+}
+
+__global__ void cc_ctrl_init_kernel(Ctrl *c) {
+  const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x, nt = (size_t)gridDim.x * blockDim.x;
+  ctrl_reset_barrier(c, t, nt);
+  if (t == 0) {
+    c->abort = kAbortNone;
+    c->checkin = 0;
+    c->resident_seen = 0;
+    c->progress = 0;
+    Sched s;
+    memset(&s, 0, sizeof s);
+    c->s = s;
   }
 }
 
@@ -133,15 +168,6 @@ __global__ void cc_count_kernel(const uint32_t *label, uint32_t V, uint32_t *ctl
   if (lane_id() == 0 && n) atomicAdd(&s_n, n);
   __syncthreads();
   if (threadIdx.x == 0 && s_n) atomic_add(ctl + kCtlCount, s_n);
-}
-
-// After each pass (one thread): count the pass, and either clear the changed
-// flag for the next pass or, when the pass hooked nothing, raise done.
-__global__ void cc_step_kernel(uint32_t *ctl) {
-  if (ctl[kCtlDone]) return;
-  ctl[kCtlPasses] += 1;
-  if (ctl[kCtlChanged]) ctl[kCtlChanged] = 0;
-  else ctl[kCtlDone] = 1;
 }
 
 }  // namespace lsg
@@ -167,42 +193,55 @@ int lsg_cc(const lsg_graph *g, int32_t *label_dev, uint32_t *ncomponents, lsg_st
   const uint32_t V = g->nnodes, E = g->nedges;
   uint32_t *label = (uint32_t *)label_dev;
   const uint32_t vgrid = (uint32_t)std::min<uint64_t>(((uint64_t)V + 255) / 256, 16ull * w->cus);
-  const uint32_t hgrid = (uint32_t)std::min<uint64_t>(((uint64_t)E / kHookRun + 256) / 256, 16ull * w->cus);
-  HIPCHK(hipEventRecord(w->ev0, w->stream));
-  hipLaunchKernelGGL(cc_init_kernel, dim3(vgrid), dim3(256), 0, w->stream, label, V, ctl);
-  uint32_t *c = w->pinned;
-  for (uint64_t enqueued = 0;;) {
-    for (uint32_t k = 0; k < kPassBatch; ++k) {
-      hipLaunchKernelGGL(cc_hook_kernel, dim3(hgrid), dim3(256), 0, w->stream, g->row_start, g->edge_dst, V, E,
-                         label, ctl);
-      hipLaunchKernelGGL(cc_jump_kernel, dim3(vgrid), dim3(256), 0, w->stream, label, V, ctl);
-      hipLaunchKernelGGL(cc_step_kernel, dim3(1), dim3(1), 0, w->stream, ctl);
-    }
-    HIPCHK(hipGetLastError());
-    enqueued += kPassBatch;
-    HIPCHK(hipMemcpyAsync(c, ctl + kCtlDone, 4, hipMemcpyDeviceToHost, w->stream));
-    HIPCHK(hipStreamSynchronize(w->stream));
-    if (c[0]) break;
-    // A pass that changes anything lowers a label and labels never rise, so a
-    // fixpoint always comes; this only guards against a broken invariant.
-    if (enqueued > (uint64_t)V + 2) return fail(LSG_ERR_TIMEOUT, "cc: no fixpoint after %llu passes",
-                                                (unsigned long long)enqueued);
-  }
-  hipLaunchKernelGGL(cc_count_kernel, dim3(vgrid), dim3(256), 0, w->stream, label, V, ctl);
-  HIPCHK(hipEventRecord(w->ev1, w->stream));
-  HIPCHK(hipMemcpyAsync(c, ctl, 16, hipMemcpyDeviceToHost, w->stream));
-  HIPCHK(hipStreamSynchronize(w->stream));
-  const uint32_t iters = c[kCtlPasses];
+  const void *fn = (const void *)cc_persistent_kernel;
+  int grid = 0;
+  if ((rc = resident_grid(w, fn, kCcBlock, &grid))) return rc;
+  grid = std::min<int>(grid, 4 * (int)w->cus);
+  CcArgs a;
+  a.row_start = g->row_start;
+  a.edge_dst = g->edge_dst;
+  a.label = label;
+  a.ctrl = w->ctrl;
+  a.V = V;
+  a.E = E;
+  // A pass that changes anything lowers a label and labels never rise, so a
+  // fixpoint always comes; the bound only guards against a broken invariant.
+  a.max_passes = V + 2;
+  a.spin_limit = spin_limit_ticks(w);
+  a.residency_limit = residency_limit_ticks(w);
+  Ctrl hc;
   float ms = 0;
-  HIPCHK(hipEventElapsedTime(&ms, w->ev0, w->ev1));
+  uint32_t *c = w->pinned;
+  for (int attempt = 0;; ++attempt) {
+    HIPCHK(hipEventRecord(w->ev0, w->stream));
+    hipLaunchKernelGGL(cc_init_kernel, dim3(vgrid), dim3(256), 0, w->stream, label, V, ctl);
+    hipLaunchKernelGGL(cc_ctrl_init_kernel, dim3(1), dim3(256), 0, w->stream, w->ctrl);
+    hipLaunchKernelGGL(cc_persistent_kernel, dim3(grid), dim3(kCcBlock), 0, w->stream, a);
+    hipLaunchKernelGGL(cc_count_kernel, dim3(vgrid), dim3(256), 0, w->stream, label, V, ctl);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(w->ev1, w->stream));
+    HIPCHK(hipMemcpyAsync(&hc, w->ctrl, kCtrlHostBytes, hipMemcpyDeviceToHost, w->stream));
+    HIPCHK(hipMemcpyAsync(c, ctl, 16, hipMemcpyDeviceToHost, w->stream));
+    HIPCHK(hipStreamSynchronize(w->stream));
+    HIPCHK(hipEventElapsedTime(&ms, w->ev0, w->ev1));
+    if (hc.abort != kAbortResidency) break;
+    if (attempt > 0 || hc.resident_seen == 0 || hc.resident_seen >= (uint32_t)grid)
+      return fail(LSG_ERR_HIP, "cc: persistent grid of %d x %d blocks is not co-resident (%u were)", grid,
+                  (int)kCcBlock, hc.resident_seen);
+    grid = (int)hc.resident_seen;
+    record_resident_grid(w, fn, kCcBlock, grid);
+  }
+  if (hc.abort == kAbortTimeout) return fail(LSG_ERR_TIMEOUT, "cc: grid barrier timed out after %u passes", hc.s.round);
+  if (!hc.s.done) return fail(LSG_ERR_TIMEOUT, "cc: no fixpoint after %u passes", hc.s.round);
+  const uint32_t iters = hc.s.round;
   if (ncomponents) *ncomponents = c[kCtlCount];
   if (stats) {
     memset(stats, 0, sizeof *stats);
     stats->ms_kernel = ms;
     stats->iterations = iters;
     stats->items = c[kCtlCount];
-    stats->grid_blocks = (int32_t)hgrid;
-    stats->block_size = 256;
+    stats->grid_blocks = grid;
+    stats->block_size = kCcBlock;
     stats->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }
   return LSG_OK;
