@@ -54,6 +54,7 @@ namespace lsg {
 constexpr uint32_t kWaveDeg = 32;  // in-degree from which a wavefront sums a vertex
 
 constexpr uint32_t kChunk = 4096;  // in-edges per wavefront-sized partial sum
+constexpr int kUnrollPr = 8;       // in-edges per lane in flight in a partial sum
 
 struct PrArgs {
   const uint32_t *row_start;         // forward CSR (out-degrees)
@@ -197,17 +198,20 @@ __global__ void pr_chunk_kernel(PrArgs a) {
     const uint32_t v = a.heavy[i];
     const uint32_t e0 = a.in_row[v] + (uint32_t)(k - a.hbase[i]) * kChunk;
     const uint32_t e1 = min(a.in_row[v + 1], e0 + kChunk);
-    // lane l sums in-edges e0 + l, e0 + l + 64, ... in that order, four per step
+    // lane l sums in-edges e0 + l, e0 + l + 64, ... in that order, eight per
+    // step.  Every lane loads (past the chunk's end it reloads the last edge
+    // and adds 0): loads under per-lane conditions make the compiler wait for
+    // the whole memory queue at each, one round trip per load.
     float sum = 0.0f;
-    for (uint32_t e = e0 + lane; e < e1; e += 4 * kWave) {
-      uint32_t u[4];
-      float c[4];
+    for (uint32_t b = e0; b < e1; b += kUnrollPr * kWave) {
+      uint32_t u[kUnrollPr];
+      float c[kUnrollPr];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) u[q] = e + q * kWave < e1 ? a.in_src[e + q * kWave] : 0u;
+      for (int q = 0; q < kUnrollPr; ++q) u[q] = a.in_src[min(b + q * kWave + lane, e1 - 1)];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) c[q] = e + q * kWave < e1 ? a.c_cur[u[q]] : 0.0f;
+      for (int q = 0; q < kUnrollPr; ++q) c[q] = a.c_cur[u[q]];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) sum += c[q];
+      for (int q = 0; q < kUnrollPr; ++q) sum += b + q * kWave + lane < e1 ? c[q] : 0.0f;
     }
 #pragma unroll
     for (int d = kWave / 2; d >= 1; d >>= 1) sum += __shfl_xor(sum, d);
