@@ -221,8 +221,11 @@ int lsg_sssp_dist_noalloc(const lsg_graph *gg, const lsg_opts *opts, int32_t *di
  * direction (weakly connected components).  label_dev[v] gets the smallest
  * vertex id of v's component; *ncomponents (may be NULL) the number of
  * components.  The reference's labels depend on its racing hooks; the
- * partition is the same.  stats: iterations = hook passes, items =
- * components. */
+ * partition is the same.  All hook and jump passes run in one persistent
+ * launch behind the leader grid barrier (the reference's gg_main_pipe_4
+ * loop, apps/cc/kernel.cu:465-707); a barrier that times out returns
+ * LSG_ERR_TIMEOUT.  stats: iterations = hook passes, items = components,
+ * grid_blocks = the persistent grid. */
 int lsg_cc(const lsg_graph *g, int32_t *label_dev, uint32_t *ncomponents, lsg_stats *stats);
 /* gg_main of apps/cc: labels into gg->node_data ("components: %d" is
  * stats->items, apps/cc/kernel.cu:719).  `hg` may be NULL. */
