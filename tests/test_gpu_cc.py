@@ -105,3 +105,25 @@ def test_cc_gg_main_and_cli(tmp_path):
         for j in range(row[i], row[i + 1]):
             want.append("\te %d: %d %d %d\n" % (j, col[j], w[j], ref[col[j]]))
     assert open(out).read() == "".join(want)
+
+
+def test_cc_barrier_timeout_is_reported(monkeypatch):
+    """A zero spin limit, set from the host, makes the persistent launch's grid barriers give
+    up at once: every wave still exits and the call reports LSG_ERR_TIMEOUT instead of hanging;
+    the next call with the default limit gives the oracle's labels."""
+    g = random_graph(4096, 16384, 11)
+    ref, _ = oracle.cc(g.row_start, g.edge_dst)
+    dev = g.copy_to_gpu()
+    lab = DeviceArray(g.nnodes * 4)
+    try:
+        monkeypatch.setenv("LSG_BARRIER_TIMEOUT_S", "0")
+        say("cc forced barrier timeout")
+        with pytest.raises(LsgError) as e:
+            cc_into(dev, lab.ptr)
+        assert e.value.name == "LSG_ERR_TIMEOUT"
+        monkeypatch.delenv("LSG_BARRIER_TIMEOUT_S")
+        cc_into(dev, lab.ptr)
+        assert np.array_equal(lab.to_numpy(np.int32, g.nnodes), ref)
+    finally:
+        lab.close()
+        dev.close()
