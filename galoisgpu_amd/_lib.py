@@ -71,7 +71,8 @@ class LsgOpts(ctypes.Structure):
         ("trace", c_int32),
         ("graph_launch", c_int32),
         ("lb_deg", c_int32),
-        ("reserved", c_int32 * 3),
+        ("single_block", c_int32),
+        ("reserved", c_int32 * 2),
     ]
 
 

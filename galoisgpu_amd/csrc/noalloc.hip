@@ -189,8 +189,8 @@ int lsg_sssp_dist_noalloc(const lsg_graph *gg, const lsg_opts *opts, int32_t *di
     return fail(LSG_ERR_ARG, "start node %d out of range [0, %u)", o.start_node, gg->nnodes);
   int bs, expand, rc;
   if ((rc = check_single_opts(o, true, &bs, &expand))) return rc;
-  if (o.trace || o.graph_launch || o.enable_lb)
-    return fail(LSG_ERR_ARG, "trace, graph_launch and -l need workspace buffers; use lsg_sssp_dist");
+  if (o.trace || o.graph_launch || o.enable_lb || o.single_block)
+    return fail(LSG_ERR_ARG, "trace, graph_launch, -l and single_block need workspace buffers; use lsg_sssp_dist");
   NoAllocDevice *nd;
   if ((rc = noalloc_device(gg->device, &nd))) return rc;
   std::lock_guard<std::mutex> lk(nd->mu);

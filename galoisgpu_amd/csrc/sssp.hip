@@ -155,8 +155,14 @@ static int run_single(const lsg_graph *g, const lsg_opts *o_in, uint32_t *dist, 
   HIPCHK(hipSetDevice(w->device));
   CsrView v{g->row_start, g->edge_dst, (const uint32_t *)g->edge_data, g->nnodes, g->nedges};
   uint32_t seed = (uint32_t)o.start_node;
-  rc = o.enable_lb ? run_lb_single((int)A, w, v, seed, o, dist, st)
-                   : run_core<A>(w, v, &seed, 1, o, dist, o.graph_launch != 0, st);
+  if (o.single_block && o.enable_lb) return fail(LSG_ERR_ARG, "single_block and enable_lb (-l) exclude each other");
+  if (o.single_block) {
+    rc = run_block_single((int)A, w, v, seed, o, dist, st);
+    if (rc == kBlockFallback) rc = run_core<A>(w, v, &seed, 1, o, dist, o.graph_launch != 0, st);
+  } else {
+    rc = o.enable_lb ? run_lb_single((int)A, w, v, seed, o, dist, st)
+                     : run_core<A>(w, v, &seed, 1, o, dist, o.graph_launch != 0, st);
+  }
   if (rc == LSG_OK && st)
     st->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return rc;

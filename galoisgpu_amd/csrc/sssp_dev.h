@@ -956,4 +956,10 @@ inline int setup_single(Workspace *w, const CsrView &g, uint32_t nseeds, const l
 int run_lb_single(int algo, Workspace *w, const CsrView &g, uint32_t seed, const lsg_opts &o, uint32_t *dist,
                   lsg_stats *st);
 
+// The one-workgroup asynchronous traversal (block.hip); kBlockFallback when
+// its LDS ring overflowed and the persistent kernel must run instead.
+constexpr int kBlockFallback = 1;
+int run_block_single(int algo, Workspace *w, const CsrView &g, uint32_t seed, const lsg_opts &o, uint32_t *dist,
+                     lsg_stats *st);
+
 }  // namespace lsg

@@ -100,7 +100,14 @@ typedef struct lsg_opts {
                            the arguments stay the same) */
   int32_t lb_deg;      /* -l: degree threshold of the edge-balanced pass; 0 = the sweep's total
                           thread count, the reference's DEGREE_LIMIT (rt/include/ggcuda.h:19-22) */
-  int32_t reserved[3];
+  int32_t single_block; /* 1 = one workgroup works through an LDS queue of vertices with no rounds
+                           (label-correcting, same distances), for small sparse high-diameter
+                           graphs, where level-synchronous rounds are latency-bound: BFS on the
+                           256x256 grid 3.6 -> 2.0 ms; SSSP there relaxes ~5x the vertices of the
+                           near/far piles and is slower (8.2 -> 23 ms).  A queue that outgrows the
+                           LDS ring falls back to the persistent kernel.  Single-source
+                           lsg_sssp_dist / lsg_bfs_dist / gg_main only; stats report items */
+  int32_t reserved[2];
 } lsg_opts;
 
 /* One record per round of a single-source traversal, written when
@@ -214,7 +221,7 @@ int lsg_bfs_dist(const lsg_graph *gg, const lsg_opts *opts, int32_t *level_dev, 
  * about 22 MB that each GPU holds until the process ends, so graphs are
  * limited to 2^20 vertices and 2^24 edges (LSG_ERR_ARG beyond).  Calls share
  * that array: calls that overlap on one GPU race.  Runs on the device's null
- * stream; trace, graph_launch and enable_lb are rejected (they need
+ * stream; trace, graph_launch, enable_lb and single_block are rejected (they need
  * workspace buffers).  Same distances as lsg_sssp_dist, bit for bit. */
 int lsg_sssp_dist_noalloc(const lsg_graph *gg, const lsg_opts *opts, int32_t *dist_dev, lsg_stats *stats);
 /* Connected components (apps/cc/kernel.cu:630-723), edges taken without

@@ -1,0 +1,103 @@
+"""GPU parity of the one-workgroup queue traversal (lsg_opts.single_block).
+
+It relaxes vertices in a different order from the near/far piles
+(label-correcting), so only the fixpoint is shared: distances must equal the
+oracle's Dijkstra / BFS bit for bit, on the golden graphs, the 256x256 grid of
+configuration 1, road-like and RMAT graphs, and a tree whose queue outgrows the
+LDS ring (the call then runs the persistent kernel and is still exact).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from galoisgpu_amd import CSRGraph, Options
+from galoisgpu_amd.apps import bfs_into, sssp_into
+from galoisgpu_amd.batch import DeviceArray
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-test]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def run(g, src, bfs=False, **kw):
+    dev = g.copy_to_gpu()
+    buf = DeviceArray(g.nnodes * 4)
+    try:
+        o = Options(start_node=src, single_block=True, **kw)
+        st = (bfs_into if bfs else sssp_into)(dev, buf.ptr, o)
+        return buf.to_numpy(np.int32, g.nnodes), st
+    finally:
+        buf.close()
+        dev.close()
+
+
+def test_single_block_golden(small_graphs):
+    for name, gg in small_graphs.items():
+        g = CSRGraph.from_arrays(gg["row"], gg["col"], gg["w"])
+        say("single block golden", name)
+        d, st = run(g, gg["src"], delta=32)
+        assert np.array_equal(d, gg["dist"]), name
+        assert st["grid_blocks"] == 1
+        b, _ = run(g, gg["src"], bfs=True)
+        assert np.array_equal(b, gg["bfs"]), name
+
+
+def test_single_block_grid256(grid256_golden):
+    g = CSRGraph.grid(256, 256, 1, 100, 1)
+    say("single block grid256")
+    d, st = run(g, 0, delta=32)
+    assert np.array_equal(d, grid256_golden["dist"])
+    assert st["grid_blocks"] == 1
+    b, _ = run(g, 0, bfs=True)
+    assert np.array_equal(b, grid256_golden["bfs"])
+
+
+@pytest.mark.parametrize("rows,seed", [(256, 3), (512, 4)])
+def test_single_block_roadlike(rows, seed):
+    g = CSRGraph.roadlike(rows, rows, 0.10, 0.001, 1, 10000, seed=seed)
+    ref = oracle.dijkstra(g.row_start, g.edge_dst, g.edge_data, 0)
+    say("single block road", rows)
+    d, _ = run(g, 0, delta=10000)
+    assert np.array_equal(d, ref)
+
+
+def test_single_block_rmat_and_zero_weights():
+    g = CSRGraph.rmat(14, 8, seed=9)
+    src = g.max_degree_vertex()
+    ref = oracle.dijkstra(g.row_start, g.edge_dst, g.edge_data, src)
+    say("single block rmat14")
+    d, _ = run(g, src, delta=32)
+    assert np.array_equal(d, ref)
+    lref, _ = oracle.bfs(g.row_start, g.edge_dst, src)
+    b, _ = run(g, src, bfs=True)
+    assert np.array_equal(b, lref)
+    # zero weights: every reachable vertex at distance 0
+    z = CSRGraph.from_arrays(g.row_start, g.edge_dst, np.zeros(g.nedges, dtype=np.int64))
+    d0, _ = run(z, src, delta=32)
+    zref = oracle.dijkstra(z.row_start, z.edge_dst, z.edge_data, src)
+    assert np.array_equal(d0, zref)
+
+
+def test_single_block_falls_back_when_the_queue_outgrows_lds():
+    """A complete 4-ary tree of depth 8 queues its 65536 leaves while its 16384
+    vertices of depth 7 are still being expanded, beyond the LDS ring: the call
+    runs the persistent kernel instead and is still exact."""
+    n = sum(4 ** k for k in range(9))  # 87381 vertices
+    deg = np.zeros(n, dtype=np.int64)
+    internal = (n - 4 ** 8)
+    deg[:internal] = 4
+    row = np.zeros(n + 1, dtype=np.int64)
+    row[1:] = np.cumsum(deg)
+    dst = np.arange(1, n)  # children of v are 4v+1 .. 4v+4
+    w = np.random.default_rng(2).integers(1, 100, n - 1)
+    g = CSRGraph.from_arrays(row, dst, w)
+    ref = oracle.dijkstra(g.row_start, g.edge_dst, g.edge_data, 0)
+    say("single block tree fallback")
+    d, st = run(g, 0, delta=32)
+    assert np.array_equal(d, ref)
+    assert st["grid_blocks"] > 1  # the persistent kernel ran
