@@ -100,13 +100,13 @@ typedef struct lsg_opts {
                            the arguments stay the same) */
   int32_t lb_deg;      /* -l: degree threshold of the edge-balanced pass; 0 = the sweep's total
                           thread count, the reference's DEGREE_LIMIT (rt/include/ggcuda.h:19-22) */
-  int32_t single_block; /* 1 = one workgroup works through an LDS queue of vertices with no rounds
-                           (label-correcting, same distances), for small sparse high-diameter
-                           graphs, where level-synchronous rounds are latency-bound: BFS on the
-                           256x256 grid 3.6 -> 2.0 ms; SSSP there relaxes ~5x the vertices of the
-                           near/far piles and is slower (8.2 -> 23 ms).  A queue that outgrows the
-                           LDS ring falls back to the persistent kernel.  Single-source
-                           lsg_sssp_dist / lsg_bfs_dist / gg_main only; stats report items */
+  int32_t single_block; /* 1 = the near/far rounds run in one workgroup with the piles in LDS and
+                           a workgroup barrier between rounds (same distances), for small sparse
+                           high-diameter graphs, where a round of the persistent kernel is a chain
+                           of global round trips: SSSP on the 256x256 grid 8.0 -> 4.2 ms at delta
+                           32, 2.5 ms at delta 256; BFS 3.6 -> 1.5 ms.  Piles that outgrow LDS
+                           (2048 near, 4096 far entries) fall back to the persistent kernel.
+                           Single-source lsg_sssp_dist / lsg_bfs_dist / gg_main only */
   int32_t reserved[2];
 } lsg_opts;
 

@@ -1,10 +1,13 @@
-"""GPU parity of the one-workgroup queue traversal (lsg_opts.single_block).
+"""GPU parity of the one-workgroup near/far traversal (lsg_opts.single_block).
 
-It relaxes vertices in a different order from the near/far piles
-(label-correcting), so only the fixpoint is shared: distances must equal the
-oracle's Dijkstra / BFS bit for bit, on the golden graphs, the 256x256 grid of
-configuration 1, road-like and RMAT graphs, and a tree whose queue outgrows the
-LDS ring (the call then runs the persistent kernel and is still exact).
+Its piles live in LDS and carry stale entries that are skipped, so its item
+counts differ from the persistent kernel's; distances must equal the oracle's
+Dijkstra / BFS bit for bit: on the golden graphs, the 256x256 grid of
+configuration 1 at delta 32 and at the tuned delta, road-like and RMAT graphs
+(wavefront-expanded heavy vertices), a graph whose far pile passes half full
+(stale far entries dropped at the split) and whose rounds exceed one pass of
+the workgroup, and a tree whose piles outgrow LDS (the call then runs the
+persistent kernel and is still exact).
 """
 import sys
 
@@ -47,14 +50,39 @@ def test_single_block_golden(small_graphs):
         assert np.array_equal(b, gg["bfs"]), name
 
 
-def test_single_block_grid256(grid256_golden):
+@pytest.mark.parametrize("delta", [32, 256, 512])
+def test_single_block_grid256(grid256_golden, delta):
     g = CSRGraph.grid(256, 256, 1, 100, 1)
-    say("single block grid256")
-    d, st = run(g, 0, delta=32)
+    say("single block grid256 delta", delta)
+    d, st = run(g, 0, delta=delta)
     assert np.array_equal(d, grid256_golden["dist"])
     assert st["grid_blocks"] == 1
-    b, _ = run(g, 0, bfs=True)
+    assert st["items"] >= g.nnodes  # every vertex expanded at least once
+    b, bst = run(g, 0, bfs=True)
     assert np.array_equal(b, grid256_golden["bfs"])
+    assert bst["near_iters"] == int(grid256_golden["bfs"].max()) + 1  # one round per level
+
+
+def test_single_block_far_prune_and_wide_rounds():
+    """Source 0 reaches 1500 leaves directly (weights 1000-1099) and through
+    vertex 1 (weights 1 + 1).  At delta 1 the far pile first holds the direct
+    entries, then also the lower ones: at the next split it is past half full
+    and the direct entries, now stale, are dropped.  The leaves' round has 1500
+    entries, more than one pass of the workgroup; 0 and 1 are wavefront-expanded."""
+    n_leaf = 1500
+    leaves = np.arange(2, 2 + n_leaf)
+    row = np.zeros(n_leaf + 3, dtype=np.int64)
+    row[1] = 1 + n_leaf
+    row[2:] = 1 + 2 * n_leaf
+    dst = np.concatenate([[1], leaves, leaves])
+    w = np.concatenate([[1], 1000 + np.arange(n_leaf) % 100, np.ones(n_leaf, dtype=np.int64)])
+    g = CSRGraph.from_arrays(row, dst, w)
+    ref = oracle.dijkstra(g.row_start, g.edge_dst, g.edge_data, 0)
+    assert (ref[2:] == 2).all()
+    say("single block far prune")
+    d, st = run(g, 0, delta=1)
+    assert np.array_equal(d, ref)
+    assert st["grid_blocks"] == 1
 
 
 @pytest.mark.parametrize("rows,seed", [(256, 3), (512, 4)])
@@ -83,10 +111,9 @@ def test_single_block_rmat_and_zero_weights():
     assert np.array_equal(d0, zref)
 
 
-def test_single_block_falls_back_when_the_queue_outgrows_lds():
-    """A complete 4-ary tree of depth 8 queues its 65536 leaves while its 16384
-    vertices of depth 7 are still being expanded, beyond the LDS ring: the call
-    runs the persistent kernel instead and is still exact."""
+def test_single_block_falls_back_when_the_piles_outgrow_lds():
+    """A complete 4-ary tree of depth 8: its deepest levels are wider than the
+    LDS piles, so the call runs the persistent kernel instead and is still exact."""
     n = sum(4 ** k for k in range(9))  # 87381 vertices
     deg = np.zeros(n, dtype=np.int64)
     internal = (n - 4 ** 8)
