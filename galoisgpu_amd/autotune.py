@@ -12,7 +12,8 @@ architecture: ``tuning/<arch>/<workload>.json`` (for example
 Workloads are the benchmark configurations, each with its own search space:
 
   grid256        plain SSSP, 256^2 grid, source 0, delta 32 (fixed): expansion
-                 mode, block size, heavy-chunk size, far-pile mode
+                 mode, block size, heavy-chunk size, far-pile mode; and the
+                 one-workgroup mode (single_block) over delta
   rmat20         plain SSSP and BFS, RMAT-20 from the max-degree vertex, delta 32
   road2048       plain SSSP, 2048^2 road-like grid, source 0, delta 10000
   rmat22_64src   batched SSSP, RMAT-22, 64 uniform sources: delta, far-pile
@@ -60,6 +61,11 @@ PLAIN_LAUNCH_SPACE = {
     "block_size": [256, 512],
     "heavy_deg": [512, 4096],
     "far_mode": [0, 2],
+}
+# The one-workgroup mode of small high-diameter graphs: its rounds are cheap,
+# so delta trades rounds against re-expanded vertices.
+SINGLE_SPACE = {
+    "delta": [32, 64, 128, 256, 512, 1024],
 }
 BFS_SPACE = {
     "expand_mode": [0, 1],
@@ -229,6 +235,10 @@ def _tune_workload(name: str, log) -> dict:
         if name == "rmat20":
             bfs = tune_single(dev, [src], BFS_SPACE, bfs=True, repeats=3, log=log)
             res.update(bfs=_best(bfs), bfs_ms=bfs[0]["ms"], bfs_results=bfs)
+        if name == "grid256":
+            single = tune_single(dev, [src], SINGLE_SPACE, fixed={"single_block": True}, repeats=3, log=log)
+            res.update(single=dict(_best(single), single_block=True), single_ms=single[0]["ms"],
+                       single_results=single)
         dev.close()
     elif name == "union256":
         graphs, srcs = _union_graphs()
@@ -256,12 +266,12 @@ def run(workload: str, out: Optional[str] = None, log=print, outdir: Optional[st
     os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
     with open(path, "w") as f:
         json.dump(res, f, indent=1)
-    log(f"{workload}: {({k: res[k] for k in res if k in ('batched', 'plain', 'bfs', 'union')})} -> {path}")
+    log(f"{workload}: {({k: res[k] for k in res if k in ('batched', 'plain', 'bfs', 'union', 'single')})} -> {path}")
     return res
 
 
 def tuned_options(name: str, which: str, device: int = 0, root: Optional[str] = None) -> Optional[dict]:
-    """Best options of kind `which` ('batched', 'plain', 'bfs' or 'union') for
+    """Best options of kind `which` ('batched', 'plain', 'bfs', 'union' or 'single') for
     workload `name` on this device's architecture: tuning/<arch>/<name>.json,
     else tuning/<name>.json; None when neither holds it."""
     base = root or TUNING

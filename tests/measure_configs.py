@@ -7,7 +7,8 @@ plain kernel), GTEPS = sum E_r / time, algorithmic bytes 12 E_r + 16 V_r per
 traversal over the kernel time as a fraction of the 8 TB/s HBM peak, and the
 oracle's serial near/far restatement timed beside it on the host.
 
-  C1  256x256 grid, w in [1,100], source 0, delta 32
+  C1  256x256 grid, w in [1,100], source 0, delta 32; also the one-workgroup
+      mode (single_block) at delta 32 and at the autotuner's delta, and BFS
   C2  RMAT-20 ef16, w in [1,255], max-degree source, delta 32 (SSSP and BFS)
   C3  2048x2048 road-like grid (10% edges removed, 0.1% shortcuts), w in
       [1,10000], source 0, delta 10000: plain fixed delta vs learned schedule
@@ -99,15 +100,26 @@ def c1(repeats: int) -> dict:
     buf = DeviceArray(g.nnodes * 4)
     er, vr = reach(g, ref)
     out = {"config": "C1", "graph": "grid256 w[1,100]", "V": g.nnodes, "E": g.nedges, "E_r": er}
+    from galoisgpu_amd import autotune
+    tuned = autotune.tuned_options("grid256", "single") or {}
+    tuned_delta = int(tuned.get("delta", 256))
     variants = [("plain", {})]
     if have_model("plain"):
         variants.append(("learned", {"sched": 1}))
+    variants += [("single_block", {"single_block": True}), ("single_block_tuned", {"single_block": True, "delta": tuned_delta})]
     for name, kw in variants:
-        o = Options(delta=32, start_node=0, **kw)
+        kw = dict({"delta": 32}, **kw)
+        o = Options(start_node=0, **kw)
         st = sssp_into(dev, buf.ptr, o)
         assert np.array_equal(buf.to_numpy(np.int32, g.nnodes), ref), f"C1 {name} distances differ from Dijkstra"
         out[name] = dict(rate(er, vr, timed_plain(dev, buf, o, repeats)), phases=st["iterations"],
-                         rounds=st["near_iters"])
+                         rounds=st["near_iters"], delta=kw["delta"], blocks=st["grid_blocks"])
+    lref, _ = oracle.bfs(g.row_start, g.edge_dst, 0)
+    for name, kw in (("bfs", {}), ("bfs_single_block", {"single_block": True})):
+        o = Options(start_node=0, **kw)
+        bfs_into(dev, buf.ptr, o)
+        assert np.array_equal(buf.to_numpy(np.int32, g.nnodes), lref), f"C1 {name} levels differ from BFS"
+        out[name] = rate(er, vr, timed_plain(dev, buf, o, repeats, bfs=True), bfs=True)
     out["cpu"] = cpu_nearfar(g, [0], 32)
     buf.close()
     dev.close()
