@@ -31,6 +31,7 @@ enum BufSlot {
   kBufUnion,
   kBufLb,
   kBufValidate,
+  kBufGraphPiles,
   kBufCount
 };
 

@@ -403,6 +403,17 @@ int lsg_sssp_graphs(const lsg_graph *graphs, uint32_t ngraphs, const uint32_t *s
   if (rc) return rc;
   std::lock_guard<std::mutex> lk(w->mu);
   HIPCHK(hipSetDevice(device));
+  if (o.single_block) {
+    // one workgroup per graph; a pile overflow runs the union launch instead
+    if (o.sched || o.trace || o.enable_lb || o.graph_launch)
+      return fail(LSG_ERR_ARG, "single_block graphs run without sched, trace, enable_lb and graph_launch");
+    rc = run_graphs_block(w, graphs, ngraphs, srcs, dist_out, o, stats);
+    if (rc != kBlockFallback) {
+      if (rc == LSG_OK && stats)
+        stats->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+      return rc;
+    }
+  }
   size_t mem_bytes = ngraphs * sizeof(UnionMember);
   size_t bytes = (V + 1) * 4 + E * 8 + V * 4 + mem_bytes + 256;
   void *ub;

@@ -956,10 +956,13 @@ inline int setup_single(Workspace *w, const CsrView &g, uint32_t nseeds, const l
 int run_lb_single(int algo, Workspace *w, const CsrView &g, uint32_t seed, const lsg_opts &o, uint32_t *dist,
                   lsg_stats *st);
 
-// The one-workgroup asynchronous traversal (block.hip); kBlockFallback when
-// its LDS ring overflowed and the persistent kernel must run instead.
+// The one-workgroup near/far traversals (block.hip).  kBlockFallback: a pile
+// overflowed and the persistent kernel (for a batch of graphs: the union
+// launch) must run instead.
 constexpr int kBlockFallback = 1;
 int run_block_single(int algo, Workspace *w, const CsrView &g, uint32_t seed, const lsg_opts &o, uint32_t *dist,
                      lsg_stats *st);
+int run_graphs_block(Workspace *w, const lsg_graph *graphs, uint32_t ngraphs, const uint32_t *srcs,
+                     int32_t *const *dist_out, const lsg_opts &o, lsg_batch_stats *stats);
 
 }  // namespace lsg

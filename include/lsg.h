@@ -294,7 +294,14 @@ int lsg_sssp_batch(const lsg_graph *g, const uint32_t *srcs, uint32_t nsrc, cons
 /* Many independent device graphs (same device), one source each, in ONE
  * persistent launch: the graphs are packed into a disjoint union and run as
  * one multi-seed traversal, which gives every graph exactly its own
- * single-source distances.  dist_out[i] is a device array of graphs[i].nnodes. */
+ * single-source distances.  dist_out[i] is a device array of graphs[i].nnodes.
+ * opts.single_block = 1: one launch of one workgroup per graph instead, each
+ * running its graph's near/far rounds with workgroup barriers and piles in
+ * HBM (no pack, no grid barrier); same distances.  A pile overflow falls back
+ * to the union launch.  It suits batches of small or high-diameter graphs
+ * (256 grids of 64x64: 4.7 -> 0.47 ms; 256 RMAT graphs of scales 8-11:
+ * 0.51 -> 0.35 ms); where a few large graphs dominate, one CU each is slower
+ * (256 RMAT graphs of scales 10-16: 2.7 -> 5.3 ms). */
 int lsg_sssp_graphs(const lsg_graph *graphs, uint32_t ngraphs, const uint32_t *srcs, const lsg_opts *opts,
                     int32_t *const *dist_out, lsg_batch_stats *stats);
 /* Per-phase feature records of the last traced SSSP call (single-source or

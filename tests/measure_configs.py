@@ -215,12 +215,14 @@ def c4b(repeats: int) -> dict:
     variants = [("union", Options(delta=32)), ("union_graph_launch", Options(delta=32, graph_launch=True))]
     if have_model("plain"):
         variants.append(("union_learned", Options(delta=32, sched=1)))
+    # one workgroup per graph (single_block), at the reference delta and wider ones
+    variants += [(f"one_workgroup_per_graph_d{d}", Options(delta=d, single_block=True)) for d in (32, 128, 512)]
     for name, o in variants:
         dists, st = sssp_graphs(devs, srcs, o)
         for k in range(len(graphs)):
             assert np.array_equal(dists[k], refs[k]), f"C4B {name} graph {k} differs"
         ms = sorted(sssp_graphs(devs, srcs, o)[1]["ms_kernel"] for _ in range(repeats))[repeats // 2]
-        out[name] = rate(er, vr, ms)
+        out[name] = dict(rate(er, vr, ms), blocks=st["grid_blocks"], rounds=st["rounds"])
     out["speedup_union_vs_plain"] = round(t_plain / out["union"]["ms"], 3)
     out["cpu"] = cpu_nearfar(graphs[int(np.argmax(scales))], [srcs[int(np.argmax(scales))]], 32)
     for b in bufs:

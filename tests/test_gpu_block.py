@@ -111,6 +111,38 @@ def test_single_block_rmat_and_zero_weights():
     assert np.array_equal(d0, zref)
 
 
+def test_graphs_one_workgroup_each():
+    """lsg_sssp_graphs with single_block: one workgroup per graph, piles in HBM.  RMAT graphs of
+    uneven sizes (wavefront-expanded hubs), a 64x64 grid (many rounds), road-like graphs, a
+    one-node graph; each graph against Dijkstra, at several deltas."""
+    from galoisgpu_amd.batch import sssp_graphs
+    rng = np.random.default_rng(23)
+    graphs, srcs = [], []
+    for i in range(40):
+        g = CSRGraph.rmat(int(rng.integers(6, 15)), 8, seed=200 + i)
+        graphs.append(g)
+        srcs.append(g.max_degree_vertex() if i % 3 else int(rng.integers(0, g.nnodes)))
+    graphs.append(CSRGraph.grid(64, 64, 1, 100, seed=5))
+    srcs.append(0)
+    for k in range(3):
+        graphs.append(CSRGraph.roadlike(96, 96, 0.10, 0.001, 1, 10000, seed=30 + k))
+        srcs.append(k)
+    graphs.append(CSRGraph.from_arrays([0, 0], [], []))
+    srcs.append(0)
+    refs = [oracle.dijkstra(g.row_start, g.edge_dst, g.edge_data, s) for g, s in zip(graphs, srcs)]
+    devs = [g.copy_to_gpu() for g in graphs]
+    try:
+        for delta in (1, 32, 1 << 20):
+            say("graphs one workgroup each, delta", delta)
+            outs, st = sssp_graphs(devs, srcs, Options(delta=delta, single_block=True))
+            assert st["grid_blocks"] == len(graphs) and st["block_size"] == 1024  # no fallback
+            bad = [k for k in range(len(graphs)) if not np.array_equal(outs[k], refs[k])]
+            assert not bad, (delta, bad[:8])
+    finally:
+        for d in devs:
+            d.close()
+
+
 def test_single_block_falls_back_when_the_piles_outgrow_lds():
     """A complete 4-ary tree of depth 8: its deepest levels are wider than the
     LDS piles, so the call runs the persistent kernel instead and is still exact."""

@@ -109,10 +109,13 @@ def test_c4b_256_graphs_one_launch():
     with ThreadPoolExecutor(max_workers=16) as ex:
         refs = list(ex.map(ref, range(len(graphs))))
     for name, o in (("reference delta 32", Options(delta=32)), ("graph_launch", Options(delta=32, graph_launch=True)),
-                    ("bucket", Options(delta=32, far_mode=2))):
+                    ("bucket", Options(delta=32, far_mode=2)),
+                    ("one workgroup per graph", Options(delta=32, single_block=True))):
         say("C4B", name)
         outs, st = sssp_graphs(devs, srcs, o)
         assert st["launches"] == 1
+        if o.single_block:
+            assert st["grid_blocks"] == len(graphs)  # no fallback to the union launch
         bad = [k for k in range(len(graphs)) if not np.array_equal(outs[k], refs[k])]
         assert not bad, (name, bad[:8])
     for d in devs:
