@@ -36,6 +36,7 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -556,13 +557,17 @@ int run_block_single(int algo, Workspace *w, const CsrView &g, uint32_t seed, co
 int run_graphs_block(Workspace *w, const lsg_graph *graphs, uint32_t ngraphs, const uint32_t *srcs,
                      int32_t *const *dist_out, const lsg_opts &o, lsg_batch_stats *stats) {
   if (o.delta <= 0) return fail(LSG_ERR_ARG, "delta must be > 0 (got %d)", o.delta);
+  // LSG_GRAPHS_PILE_CAP: a smaller pile capacity, so that tests can make the
+  // piles overflow and check the fallback
+  uint64_t cap_limit = UINT32_MAX / 2;
+  if (const char *e = getenv("LSG_GRAPHS_PILE_CAP")) cap_limit = std::max<uint64_t>(strtoull(e, nullptr, 10), 64);
   std::vector<GraphJob> jobs(ngraphs);
   uint64_t off = 0;
   for (uint32_t i = 0; i < ngraphs; ++i) {
     const lsg_graph &g = graphs[i];
     // a round pile holds at most one entry per improvement of the round
     // (<= nedges) plus the source; the far pile is pruned when half full
-    const uint64_t cap = std::min<uint64_t>(g.nedges + g.nnodes + 64, UINT32_MAX / 2);
+    const uint64_t cap = std::min<uint64_t>(g.nedges + g.nnodes + 64, cap_limit);
     jobs[i] = GraphJob{g.row_start, g.edge_dst, (const uint32_t *)g.edge_data, (uint32_t *)dist_out[i], g.nedges, off,
                        g.nnodes, srcs[i], (uint32_t)cap, (uint32_t)cap};
     off += 4 * cap;

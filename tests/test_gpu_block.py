@@ -143,6 +143,31 @@ def test_graphs_one_workgroup_each():
             d.close()
 
 
+def test_graphs_one_workgroup_each_falls_back_on_overflow(monkeypatch):
+    """Piles capped at 64 entries (LSG_GRAPHS_PILE_CAP): the RMAT graphs' rounds outgrow them, the
+    call runs the union launch instead (one persistent grid) and every graph is still exact."""
+    from galoisgpu_amd.batch import sssp_graphs
+    graphs = [CSRGraph.rmat(10 + (i % 3), 8, seed=300 + i) for i in range(12)]
+    srcs = [g.max_degree_vertex() for g in graphs]
+    refs = [oracle.dijkstra(g.row_start, g.edge_dst, g.edge_data, s) for g, s in zip(graphs, srcs)]
+    devs = [g.copy_to_gpu() for g in graphs]
+    try:
+        monkeypatch.setenv("LSG_GRAPHS_PILE_CAP", "64")
+        say("graphs fallback")
+        outs, st = sssp_graphs(devs, srcs, Options(delta=32, single_block=True))
+        assert st["block_size"] != 1024 or st["grid_blocks"] != len(graphs)  # the union launch ran
+        for k in range(len(graphs)):
+            assert np.array_equal(outs[k], refs[k]), k
+        monkeypatch.delenv("LSG_GRAPHS_PILE_CAP")
+        outs, st = sssp_graphs(devs, srcs, Options(delta=32, single_block=True))
+        assert st["grid_blocks"] == len(graphs) and st["block_size"] == 1024
+        for k in range(len(graphs)):
+            assert np.array_equal(outs[k], refs[k]), k
+    finally:
+        for d in devs:
+            d.close()
+
+
 def test_single_block_falls_back_when_the_piles_outgrow_lds():
     """A complete 4-ary tree of depth 8: its deepest levels are wider than the
     LDS piles, so the call runs the persistent kernel instead and is still exact."""
