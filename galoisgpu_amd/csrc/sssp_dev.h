@@ -694,6 +694,204 @@ __device__ void leader_step(const KArgs &a, Sched &s, uint32_t &abort_seen, uint
   }
 }
 
+// ---- pile entries: block-local rounds and one-workgroup traversals --------
+//
+// A pile entry carries the vertex, the distance it was pushed with and its
+// edge range, so a round of a single workgroup starts with the edge loads; the
+// vertex's current distance is loaded beside them, and an entry whose vertex
+// has since been lowered below the entry's distance is skipped (a later entry
+// carries the lower distance; distances only fall, so a stale read never
+// skips).  The neighbours' row bounds are loaded beside the atomicMins.  A
+// round is then two dependent round trips (edges, atomicMins), against the
+// in-queue load, d[u], row bounds, edges, atomicMin and stamp exchange of a
+// round that pops vertex ids.
+
+// Vertices of more edges are expanded by their whole wavefront, 64 edges a step.
+constexpr uint32_t kEntryHeavyDeg = 32;
+
+struct alignas(16) Entry {
+  uint32_t v;   // vertex
+  uint32_t d;   // distance it was pushed with
+  uint32_t rs;  // row_start[v]
+  uint32_t re;  // row_start[v + 1]
+};
+
+// Pile pointers carry their address space, so that LDS piles are accessed
+// with LDS instructions and HBM piles with global ones.
+constexpr int kLdsAS = 3, kGlobalAS = 1;
+template <int AS>
+using EntryPtr = __attribute__((address_space(AS))) Entry *;
+template <int AS>
+__device__ __forceinline__ Entry ld_entry(EntryPtr<AS> p) {
+  return Entry{p->v, p->d, p->rs, p->re};
+}
+template <int AS>
+__device__ __forceinline__ void st_entry(EntryPtr<AS> p, const Entry &e) {
+  p->v = e.v;
+  p->d = e.d;
+  p->rs = e.rs;
+  p->re = e.re;
+}
+
+// Where a round appends: the next round's pile and the far pile, their counts
+// (in LDS) and capacities, and the overflow flag.
+template <int AS>
+struct Sink {
+  EntryPtr<AS> near;
+  uint32_t *near_count;
+  uint32_t near_cap;
+  EntryPtr<AS> far;
+  uint32_t *far_count;
+  uint32_t far_cap;
+  uint32_t *abort;
+};
+
+constexpr uint32_t kRun = 0, kSplit = 1, kDone = 2;
+
+// One workgroup issues every access of a round through one CU's vector memory
+// pipeline, which processes a scattered access lane by lane: a round costs in
+// proportion to the lane accesses it makes, so consecutive words are fetched
+// by one instruction (4-byte alignment is all that multi-word loads need).
+__device__ __forceinline__ uint64_t ld_pair(const uint32_t *p) {
+  uint64_t x;
+  __builtin_memcpy(&x, glb(p), 8);
+  return x;
+}
+__device__ __forceinline__ uint4 ld_quad(const uint32_t *p) {
+  uint4 x;
+  __builtin_memcpy(&x, glb(p), 16);
+  return x;
+}
+
+// Appends entry e[k] to the near pile where pa[k] holds and to the far pile
+// where pb[k] holds, for k < K: one reservation per pile per wavefront, the
+// wavefront's entries of k following those of 0..k-1.  Wavefront-uniform
+// control flow; an overflow raises kAbortCapacity.
+template <int K, int AS>
+__device__ __forceinline__ void push2(const bool (&pa)[K], const bool (&pb)[K], const Entry (&e)[K],
+                                      const Sink<AS> &s) {
+  uint64_t ma[K], mb[K];
+  uint32_t ta = 0, tb = 0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    ma[k] = __ballot(pa[k]);
+    mb[k] = __ballot(pb[k]);
+    ta += (uint32_t)__builtin_popcountll(ma[k]);
+    tb += (uint32_t)__builtin_popcountll(mb[k]);
+  }
+  if (ta + tb == 0) return;
+  uint32_t ba = 0, bb = 0;
+  if (lane_id() == 0) {
+    if (ta) ba = atomicAdd(s.near_count, ta);
+    if (tb) bb = atomicAdd(s.far_count, tb);
+  }
+  ba = readlane(ba, 0);
+  bb = readlane(bb, 0);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if (pa[k]) {
+      const uint32_t pos = ba + lanes_below(ma[k]);
+      if (pos < s.near_cap) st_entry<AS>(s.near + pos, e[k]);
+      else *s.abort = kAbortCapacity;
+    }
+    if (pb[k]) {
+      const uint32_t pos = bb + lanes_below(mb[k]);
+      if (pos < s.far_cap) st_entry<AS>(s.far + pos, e[k]);
+      else *s.abort = kAbortCapacity;
+    }
+    ba += (uint32_t)__builtin_popcountll(ma[k]);
+    bb += (uint32_t)__builtin_popcountll(mb[k]);
+  }
+}
+
+// Relaxes edges j .. j+3 of a light entry e (ldeg edges; du: the vertex's
+// distance as loaded) with relax(a, v, nd, act), for the whole wavefront.
+template <Algo A, class Relax>
+__device__ __forceinline__ void expand_light(const KArgs &a, const Entry &e, uint32_t du, uint32_t ldeg, uint32_t j,
+                                             const Relax &relax) {
+  uint32_t v[kInFlight], nd[kInFlight];
+  bool act[kInFlight];
+  // edges base .. base+3 of the row (past its end: other rows' edges, unused)
+  const uint32_t base = j < ldeg ? e.rs + j : 0u;
+  if (base + kInFlight <= a.nedges) {
+    const uint4 dq = ld_quad(a.edge_dst + base);
+    v[0] = dq.x, v[1] = dq.y, v[2] = dq.z, v[3] = dq.w;
+    if (A == Algo::SSSP) {
+      const uint4 wq = ld_quad(a.edge_data + base);
+      nd[0] = e.d + wq.x, nd[1] = e.d + wq.y, nd[2] = e.d + wq.z, nd[3] = e.d + wq.w;
+    }
+  } else {  // the last edges of the array
+#pragma unroll
+    for (int k = 0; k < kInFlight; ++k) {
+      const uint32_t ei = (uint32_t)min<uint64_t>(base + k, a.nedges - 1);
+      v[k] = glb(a.edge_dst)[ei];
+      if (A == Algo::SSSP) nd[k] = e.d + glb(a.edge_data)[ei];
+    }
+  }
+  if (A == Algo::BFS)
+#pragma unroll
+    for (int k = 0; k < kInFlight; ++k) nd[k] = e.d + 1u;
+  // d[u] is read only after the edge loads are issued (left to itself the
+  // compiler hoists the test, and its wait, above them)
+  uint32_t dnow = du;
+  asm volatile("" : "+v"(dnow));
+#pragma unroll
+  for (int k = 0; k < kInFlight; ++k) act[k] = j + k < ldeg && !(dnow < e.d);  // a lowered vertex has a newer entry
+  relax(a, v, nd, act);
+}
+
+// Expands the entries in[0, n) with a workgroup of NT threads: light vertices
+// one per lane, heavy ones (more than kEntryHeavyDeg edges) by their whole
+// wavefront.  relax(a, v, nd, act) relaxes kInFlight edges per lane (edge k:
+// to v[k] with candidate nd[k], where act[k]), called with the whole
+// wavefront.  Adds the live (not stale) entries to *items.  (Functors rather
+// than lambdas: a lambda holding a reference to the kernel arguments made the
+// persistent kernel copy them to scratch.)
+template <Algo A, int AS, int NT, class Relax>
+__device__ __forceinline__ void expand_entries(const KArgs &a, EntryPtr<AS> in, uint32_t n, const Relax &relax,
+                                               unsigned long long *items) {
+  const uint32_t lane = lane_id();
+  uint32_t expanded = 0;
+  for (uint32_t b0 = 0; b0 < n; b0 += NT) {
+    const uint32_t i = b0 + threadIdx.x;
+    const bool valid = i < n;
+    Entry e{0u, kInf, 0u, 0u};
+    if (valid) e = ld_entry<AS>(in + i);
+    // issued by every lane (a load under a per-lane condition makes the
+    // compiler wait for it before the edge loads); d[0] for idle lanes
+    const uint32_t du = ld_coh(a.dist + e.v);
+    const uint32_t deg = valid ? e.re - e.rs : 0u;
+    const bool heavy = deg > kEntryHeavyDeg;
+    const uint32_t ldeg = heavy ? 0u : deg;
+    // the first batch outside the loop: its edge loads go out beside the load of d[u]
+    // (loop bounds by ballot: no cross-lane reduction through LDS)
+    if (__ballot(ldeg > 0)) expand_light<A>(a, e, du, ldeg, 0u, relax);
+    for (uint32_t j = kInFlight; __ballot(j < ldeg); j += kInFlight) expand_light<A>(a, e, du, ldeg, j, relax);
+    const bool live = valid && !(du < e.d);
+    expanded += (uint32_t)__builtin_popcountll(__ballot(live));
+    uint64_t hm = __ballot(live && heavy);
+    while (hm) {
+      const uint32_t l = (uint32_t)__builtin_ctzll(hm);
+      hm &= hm - 1;
+      const uint32_t hd = readlane(e.d, l), hrs = readlane(e.rs, l), hre = readlane(e.re, l);
+      for (uint32_t j = hrs; j < hre; j += kWave * kInFlight) {
+        uint32_t v[kInFlight], nd[kInFlight];
+        bool act[kInFlight];
+#pragma unroll
+        for (int k = 0; k < kInFlight; ++k) {
+          const uint32_t idx = j + (uint32_t)k * kWave + lane;
+          act[k] = idx < hre;
+          const uint32_t ei = act[k] ? idx : hrs;
+          v[k] = glb(a.edge_dst)[ei];
+          nd[k] = hd + (A == Algo::SSSP ? glb(a.edge_data)[ei] : 1u);
+        }
+        relax(a, v, nd, act);
+      }
+    }
+  }
+  if (lane == 0 && expanded) atomicAdd(items, (unsigned long long)expanded);
+}
+
 template <int BS>
 __device__ __forceinline__ uint32_t tile_items(uint32_t in_count, uint32_t nblocks) {
   uint32_t per = (in_count + nblocks - 1) / nblocks;
