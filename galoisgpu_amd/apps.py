@@ -34,7 +34,7 @@ class Options:
     trace: bool = False
     graph_launch: bool = False
     lb_deg: int = 0             # -l threshold; 0 = the sweep's thread count (DEGREE_LIMIT)
-    single_block: bool = False  # one-workgroup queue traversal (small high-diameter graphs)
+    single_block: bool = False  # near/far rounds in one workgroup (small high-diameter graphs; per graph in sssp_graphs)
 
     def to_c(self) -> LsgOpts:
         o = LsgOpts()
