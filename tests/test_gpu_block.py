@@ -168,6 +168,40 @@ def test_graphs_one_workgroup_each_falls_back_on_overflow(monkeypatch):
             d.close()
 
 
+def test_single_block_seeded_sweep():
+    """24 seeded graphs of the three families at random sizes, sources and deltas (weights from 0,
+    delta from 1): every single-source run, and the batch of all of them, equal Dijkstra."""
+    from galoisgpu_amd.batch import sssp_graphs
+    rng = np.random.default_rng(77)
+    graphs, srcs, deltas = [], [], []
+    for i in range(24):
+        kind = i % 3
+        if kind == 0:
+            g = CSRGraph.grid(int(rng.integers(2, 90)), int(rng.integers(2, 90)), 0, int(rng.integers(1, 300)),
+                              seed=400 + i)
+        elif kind == 1:
+            n = int(rng.integers(8, 120))
+            g = CSRGraph.roadlike(n, n, 0.15, 0.002, 1, int(rng.integers(2, 20000)), seed=400 + i)
+        else:
+            g = CSRGraph.rmat(int(rng.integers(4, 13)), int(rng.integers(1, 12)), seed=400 + i)
+        graphs.append(g)
+        srcs.append(int(rng.integers(0, g.nnodes)))
+        deltas.append(int(rng.choice([1, 7, 32, 300, 5000, 1 << 24])))
+    refs = [oracle.dijkstra(g.row_start, g.edge_dst, g.edge_data, s) for g, s in zip(graphs, srcs)]
+    say("single block seeded sweep")
+    for k, (g, s, dl) in enumerate(zip(graphs, srcs, deltas)):
+        d, _ = run(g, s, delta=dl)
+        assert np.array_equal(d, refs[k]), (k, g.nnodes, dl)
+    devs = [g.copy_to_gpu() for g in graphs]
+    try:
+        outs, st = sssp_graphs(devs, srcs, Options(delta=32, single_block=True))
+        for k in range(len(graphs)):
+            assert np.array_equal(outs[k], refs[k]), k
+    finally:
+        for d in devs:
+            d.close()
+
+
 def test_single_block_falls_back_when_the_piles_outgrow_lds():
     """A complete 4-ary tree of depth 8: its deepest levels are wider than the
     LDS piles, so the call runs the persistent kernel instead and is still exact."""
