@@ -86,6 +86,11 @@ struct BlockRelax {
                                              const uint32_t (&nd)[kInFlight], const bool (&act)[kInFlight]) const {
     relax_batch<A, AS>(a, v, nd, act, thr, s);
   }
+  // Once a push has found a pile full the traversal continues elsewhere, so
+  // what is left of the round is wasted work: a heavy vertex's loop stops.
+  __device__ __forceinline__ bool stopped() const {
+    return __hip_atomic_load(s.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != kAbortNone;
+  }
 };
 
 // One round of the one-workgroup kernels: expands in[0, n), appending to `s`.
@@ -358,7 +363,7 @@ __global__ __launch_bounds__(kBlkThreads) void graphs_kernel(const GraphJob *job
 // kernel instead).
 template <Algo A>
 static int run_block(Workspace *w, const CsrView &g, const uint32_t *seeds, uint32_t nseeds, const lsg_opts &o,
-                     uint32_t *dist, lsg_stats *st) {
+                     uint32_t *dist, lsg_stats *st, int64_t *resume_cd) {
   int bs, rc;
   KArgs a;
   void *seedbuf;
@@ -376,7 +381,6 @@ static int run_block(Workspace *w, const CsrView &g, const uint32_t *seeds, uint
   HIPCHK(hipStreamSynchronize(w->stream));
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, w->ev0, w->ev1));
-  if (hc.abort == kAbortCapacity) return kBlockFallback;  // the only way it stops early
   if (st) {
     memset(st, 0, sizeof *st);
     st->ms_kernel = ms;
@@ -387,13 +391,17 @@ static int run_block(Workspace *w, const CsrView &g, const uint32_t *seeds, uint
     st->grid_blocks = 1;
     st->block_size = (int32_t)kBlkThreads;
   }
-  return LSG_OK;
+  // A pile overflow is the only way it stops early.  The distances reached so
+  // far stay in dist; the threshold it had reached (entries at or below it
+  // are near) is where the persistent kernel's curdelta continues.
+  *resume_cd = A == Algo::SSSP ? hc.s.curdelta() : 0;
+  return hc.abort == kAbortCapacity ? kBlockFallback : LSG_OK;
 }
 
 int run_block_single(int algo, Workspace *w, const CsrView &g, uint32_t seed, const lsg_opts &o, uint32_t *dist,
-                     lsg_stats *st) {
-  return algo == (int)Algo::SSSP ? run_block<Algo::SSSP>(w, g, &seed, 1, o, dist, st)
-                                 : run_block<Algo::BFS>(w, g, &seed, 1, o, dist, st);
+                     lsg_stats *st, int64_t *resume_cd) {
+  return algo == (int)Algo::SSSP ? run_block<Algo::SSSP>(w, g, &seed, 1, o, dist, st, resume_cd)
+                                 : run_block<Algo::BFS>(w, g, &seed, 1, o, dist, st, resume_cd);
 }
 
 int run_graphs_block(Workspace *w, const lsg_graph *graphs, uint32_t ngraphs, const uint32_t *srcs,

@@ -47,10 +47,12 @@ static KernelFn pick_kernel(int bs) {
 
 // Runs one traversal from `nseeds` seeds (host array) into dist (device,
 // g.nnodes entries).  Synchronous.  With graph_launch the init, seed and
-// persistent kernels are replayed as one captured hipGraph.
+// persistent kernels are replayed as one captured hipGraph.  resume_cd >= 0:
+// continue from the distances already in dist instead (every reached vertex
+// queued, curdelta resume_cd); the seeds are then not used.
 template <Algo A>
 static int run_core(Workspace *w, const CsrView &g, const uint32_t *seeds, uint32_t nseeds, const lsg_opts &o,
-                    uint32_t *dist, bool graph_launch, lsg_stats *st) {
+                    uint32_t *dist, bool graph_launch, lsg_stats *st, int64_t resume_cd = -1) {
   auto t0 = std::chrono::steady_clock::now();
   int bs, rc;
   KArgs a;
@@ -75,14 +77,19 @@ static int run_core(Workspace *w, const CsrView &g, const uint32_t *seeds, uint3
     const uint32_t init_blocks = std::min<uint32_t>((g.nnodes + 255) / 256, 8u * w->cus);
     const uint32_t *sd = (const uint32_t *)seedbuf;
     auto enqueue = [&]() -> int {
-      hipLaunchKernelGGL(init_kernel, dim3(init_blocks), dim3(256), 0, w->stream, a, a.delta);
-      hipLaunchKernelGGL(seed_kernel, dim3(1), dim3(256), 0, w->stream, a, sd, nseeds);
+      if (resume_cd >= 0) {
+        hipLaunchKernelGGL(resume_init_kernel, dim3(init_blocks), dim3(256), 0, w->stream, a, a.delta, resume_cd);
+        hipLaunchKernelGGL(resume_seed_kernel, dim3(init_blocks), dim3(256), 0, w->stream, a);
+      } else {
+        hipLaunchKernelGGL(init_kernel, dim3(init_blocks), dim3(256), 0, w->stream, a, a.delta);
+        hipLaunchKernelGGL(seed_kernel, dim3(1), dim3(256), 0, w->stream, a, sd, nseeds);
+      }
       hipLaunchKernelGGL(fn, dim3(grid), dim3(bs), 0, w->stream, a);
       HIPCHK(hipGetLastError());
       return LSG_OK;
     };
     HIPCHK(hipEventRecord(w->ev0, w->stream));
-    if (graph_launch) {
+    if (graph_launch && resume_cd < 0) {
       std::vector<uint8_t> key;
       key_append(key, (int)A);
       key_append(key, a);
@@ -157,8 +164,22 @@ static int run_single(const lsg_graph *g, const lsg_opts *o_in, uint32_t *dist, 
   uint32_t seed = (uint32_t)o.start_node;
   if (o.single_block && o.enable_lb) return fail(LSG_ERR_ARG, "single_block and enable_lb (-l) exclude each other");
   if (o.single_block) {
-    rc = run_block_single((int)A, w, v, seed, o, dist, st);
-    if (rc == kBlockFallback) rc = run_core<A>(w, v, &seed, 1, o, dist, o.graph_launch != 0, st);
+    lsg_stats bst;
+    int64_t cd = 0;
+    rc = run_block_single((int)A, w, v, seed, o, dist, &bst, &cd);
+    if (rc == kBlockFallback) {
+      // The piles overflowed: the persistent kernel continues from the
+      // distances the workgroup reached (not captured: a one-off launch).
+      rc = run_core<A>(w, v, &seed, 1, o, dist, false, st, cd);
+      if (rc == LSG_OK && st) {
+        st->ms_kernel += bst.ms_kernel;
+        st->items += bst.items;
+        st->near_iters += bst.near_iters;
+        st->iterations += bst.iterations;
+      }
+    } else if (rc == LSG_OK && st) {
+      *st = bst;
+    }
   } else {
     rc = o.enable_lb ? run_lb_single((int)A, w, v, seed, o, dist, st)
                      : run_core<A>(w, v, &seed, 1, o, dist, o.graph_launch != 0, st);

@@ -844,7 +844,8 @@ __device__ __forceinline__ void expand_light(const KArgs &a, const Entry &e, uin
 // one per lane, heavy ones (more than kEntryHeavyDeg edges) by their whole
 // wavefront.  relax(a, v, nd, act) relaxes kInFlight edges per lane (edge k:
 // to v[k] with candidate nd[k], where act[k]), called with the whole
-// wavefront.  Adds the live (not stale) entries to *items.  (Functors rather
+// wavefront; a heavy vertex's edges stop early once relax.stopped().  Adds
+// the live (not stale) entries to *items.  (Functors rather
 // than lambdas: a lambda holding a reference to the kernel arguments made the
 // persistent kernel copy them to scratch.)
 template <Algo A, int AS, int NT, class Relax>
@@ -874,7 +875,7 @@ __device__ __forceinline__ void expand_entries(const KArgs &a, EntryPtr<AS> in, 
       const uint32_t l = (uint32_t)__builtin_ctzll(hm);
       hm &= hm - 1;
       const uint32_t hd = readlane(e.d, l), hrs = readlane(e.rs, l), hre = readlane(e.re, l);
-      for (uint32_t j = hrs; j < hre; j += kWave * kInFlight) {
+      for (uint32_t j = hrs; j < hre && !relax.stopped(); j += kWave * kInFlight) {
         uint32_t v[kInFlight], nd[kInFlight];
         bool act[kInFlight];
 #pragma unroll
@@ -1008,15 +1009,10 @@ __global__ __launch_bounds__(BS, 4) void persistent_kernel(KArgs a) {
   }
 }
 
-// Initialise distances, stamps and the control block (apps/sssp/kernel.cu:23-35,469;
-// apps/bfs/kernel.cu:20-32).  The sources are seeded by seed_kernel.
-static __global__ void init_kernel(KArgs a, uint32_t delta) {
-  uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
-  for (uint32_t v = tid; v < a.nnodes; v += nt) {
-    glb(a.dist)[v] = kInf;
-    glb(a.near_stamp)[v] = 0u;
-    glb(a.far_stamp)[v] = 0u;
-  }
+// Resets the control block: empty queues, round 1, curdelta cd0 (0 for a new
+// traversal, apps/sssp/kernel.cu:472), the barrier words.  Every thread of the
+// grid calls it (the barrier words are cleared grid-strided).
+__device__ __forceinline__ void ctrl_init(const KArgs &a, uint32_t delta, int64_t cd0, uint32_t tid, uint32_t nt) {
   ctrl_reset_barrier(a.ctrl, tid, nt);
   if (tid == 0) {
     Ctrl *c = a.ctrl;
@@ -1036,12 +1032,64 @@ static __global__ void init_kernel(KArgs a, uint32_t delta) {
     s.near_q = 1;
     s.far_q = 2;
     s.round = 1;
-    s.set_curdelta(0);  // apps/sssp/kernel.cu:472
+    s.set_curdelta(cd0);
     s.delta = delta;
     s.aux[1] = a.far_mode;
     s.aux[2] = a.expand_mode;
     c->s = s;
     c->stat_items = c->stat_rounds = c->stat_heavy = c->stat_phases = c->stat_edges = 0;
+  }
+}
+
+// Initialise distances, stamps and the control block (apps/sssp/kernel.cu:23-35,469;
+// apps/bfs/kernel.cu:20-32).  The sources are seeded by seed_kernel.
+static __global__ void init_kernel(KArgs a, uint32_t delta) {
+  uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
+  for (uint32_t v = tid; v < a.nnodes; v += nt) {
+    glb(a.dist)[v] = kInf;
+    glb(a.near_stamp)[v] = 0u;
+    glb(a.far_stamp)[v] = 0u;
+  }
+  ctrl_init(a, delta, 0, tid, nt);
+}
+
+// Continuing a traversal that another kernel left part done (the one-workgroup
+// kernel whose piles overflowed): distances kept, stamps and control block
+// reset, curdelta at the threshold the traversal had reached.
+// resume_seed_kernel then queues the reached vertices.
+static __global__ void resume_init_kernel(KArgs a, uint32_t delta, int64_t cd0) {
+  uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
+  for (uint32_t v = tid; v < a.nnodes; v += nt) {
+    glb(a.near_stamp)[v] = 0u;
+    glb(a.far_stamp)[v] = 0u;
+  }
+  ctrl_init(a, delta, cd0, tid, nt);
+}
+
+// Every reached vertex (d < INF) forms the first in-queue, so the traversal
+// continues exactly: every distance is the length of a path, and every vertex
+// whose distance may not yet have been passed on to its neighbours (a pending
+// pile entry, or an improvement whose push found the pile full) is expanded
+// again.  Settled vertices are expanded once more, for nothing.  The queue
+// order is unspecified; the distances do not depend on it.  256-thread
+// blocks, grid-strided in whole blocks; one queue reservation per block step.
+static __global__ __launch_bounds__(256) void resume_seed_kernel(KArgs a) {
+  __shared__ uint32_t cnt, base;
+  Ctrl *c = a.ctrl;
+  for (uint32_t v0 = blockIdx.x * 256u; v0 < a.nnodes; v0 += gridDim.x * 256u) {
+    const uint32_t v = v0 + threadIdx.x;
+    const bool reached = v < a.nnodes && glb(a.dist)[v] != kInf;
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    const uint32_t slot = reached ? atomicAdd(&cnt, 1u) : 0u;
+    __syncthreads();
+    if (threadIdx.x == 0 && cnt) {
+      base = atomicAdd(&c->qcount[0], cnt);
+      atomicAdd(&c->s.in_count, cnt);
+    }
+    __syncthreads();
+    if (reached) glb(a.q[0])[base + slot] = v;
+    __syncthreads();  // base and cnt are read before the next step resets them
   }
 }
 
@@ -1155,11 +1203,13 @@ int run_lb_single(int algo, Workspace *w, const CsrView &g, uint32_t seed, const
                   lsg_stats *st);
 
 // The one-workgroup near/far traversals (block.hip).  kBlockFallback: a pile
-// overflowed and the persistent kernel (for a batch of graphs: the union
-// launch) must run instead.
+// overflowed.  A single source then continues in the persistent kernel from
+// the distances left in dist, with curdelta *resume_cd (run_core's
+// resume_cd); a batch of graphs runs the union launch instead.  *st holds the
+// one-workgroup launch's figures either way.
 constexpr int kBlockFallback = 1;
 int run_block_single(int algo, Workspace *w, const CsrView &g, uint32_t seed, const lsg_opts &o, uint32_t *dist,
-                     lsg_stats *st);
+                     lsg_stats *st, int64_t *resume_cd);
 int run_graphs_block(Workspace *w, const lsg_graph *graphs, uint32_t ngraphs, const uint32_t *srcs,
                      int32_t *const *dist_out, const lsg_opts &o, lsg_batch_stats *stats);
 

@@ -219,3 +219,44 @@ def test_single_block_falls_back_when_the_piles_outgrow_lds():
     d, st = run(g, 0, delta=32)
     assert np.array_equal(d, ref)
     assert st["grid_blocks"] > 1  # the persistent kernel ran
+    lref, _ = oracle.bfs(g.row_start, g.edge_dst, 0)
+    b, bst = run(g, 0, bfs=True)
+    assert np.array_equal(b, lref)
+    assert bst["grid_blocks"] > 1
+
+
+def test_single_block_overflow_continues_in_the_persistent_kernel():
+    """Graphs whose piles outgrow LDS after many rounds or after a few: the persistent kernel
+    continues from the distances the workgroup left (every reached vertex queued again), exact;
+    the rounds of both launches are counted.  Grids at wide deltas, road-like graphs at their
+    delta, RMAT from random sources, BFS on each, zero weights and delta 1 among them."""
+    rng = np.random.default_rng(91)
+    cases = [(CSRGraph.grid(512, 512, 1, 100, seed=2), 0, 256),
+             (CSRGraph.roadlike(256, 256, 0.10, 0.001, 1, 10000, seed=3), 0, 10000)]
+    for i in range(10):
+        kind = i % 3
+        if kind == 0:
+            g = CSRGraph.grid(int(rng.integers(200, 400)), int(rng.integers(200, 400)), int(rng.integers(0, 2)),
+                              100, seed=500 + i)
+            dl = int(rng.choice([512, 2048, 1 << 20]))
+        elif kind == 1:
+            n = int(rng.integers(150, 300))
+            g = CSRGraph.roadlike(n, n, 0.10, 0.001, 1, 10000, seed=500 + i)
+            dl = int(rng.choice([1, 5000, 20000]))
+        else:
+            g = CSRGraph.rmat(int(rng.integers(12, 16)), 8, seed=500 + i)
+            dl = int(rng.choice([1, 32, 1 << 20]))
+        cases.append((g, int(rng.integers(0, g.nnodes)), dl))
+    fell_back = 0
+    for k, (g, src, dl) in enumerate(cases):
+        ref = oracle.dijkstra(g.row_start, g.edge_dst, g.edge_data, src)
+        say("single block overflow case", k, g.nnodes, src, dl)
+        d, st = run(g, src, delta=dl)
+        assert np.array_equal(d, ref), (k, g.nnodes, src, dl)
+        fell_back += st["grid_blocks"] > 1
+        if st["grid_blocks"] > 1:
+            assert st["near_iters"] >= 2
+        lref, _ = oracle.bfs(g.row_start, g.edge_dst, src)
+        b, _ = run(g, src, bfs=True)
+        assert np.array_equal(b, lref), (k, "bfs")
+    assert fell_back >= len(cases) // 2, fell_back
