@@ -6,8 +6,8 @@ Dijkstra / BFS bit for bit: on the golden graphs, the 256x256 grid of
 configuration 1 at delta 32 and at the tuned delta, road-like and RMAT graphs
 (wavefront-expanded heavy vertices), a graph whose far pile passes half full
 (stale far entries dropped at the split) and whose rounds exceed one pass of
-the workgroup, and a tree whose piles outgrow LDS (the call then runs the
-persistent kernel and is still exact).
+the workgroup, and graphs whose piles outgrow LDS early or late (the
+persistent kernel then continues from the distances reached, still exact).
 """
 import sys
 
