@@ -363,7 +363,7 @@ __global__ __launch_bounds__(kBlkThreads) void graphs_kernel(const GraphJob *job
 // kernel instead).
 template <Algo A>
 static int run_block(Workspace *w, const CsrView &g, const uint32_t *seeds, uint32_t nseeds, const lsg_opts &o,
-                     uint32_t *dist, lsg_stats *st, int64_t *resume_cd) {
+                     uint32_t *dist, lsg_stats *st, Resume *resume) {
   int bs, rc;
   KArgs a;
   void *seedbuf;
@@ -391,17 +391,22 @@ static int run_block(Workspace *w, const CsrView &g, const uint32_t *seeds, uint
     st->grid_blocks = 1;
     st->block_size = (int32_t)kBlkThreads;
   }
-  // A pile overflow is the only way it stops early.  The distances reached so
-  // far stay in dist; the threshold it had reached (entries at or below it
-  // are near) is where the persistent kernel's curdelta continues.
-  *resume_cd = A == Algo::SSSP ? hc.s.curdelta() : 0;
+  // A pile overflow is the only way it stops early, at the end of the round
+  // in which it happened; the distances reached so far stay in dist.  SSSP
+  // continues with curdelta at the threshold it had reached (entries at or
+  // below it are near) from every reached vertex.  BFS continues from the
+  // level of that round (round r expands level r).
+  // (No round at all: the seeds did not fit, and level 0 is theirs.)
+  const uint32_t level = hc.stat_rounds ? (uint32_t)(hc.stat_rounds - 1) : 0u;
+  if (A == Algo::SSSP) *resume = Resume{hc.s.curdelta(), 0u, kInf - 1u};
+  else *resume = Resume{0, level, level};
   return hc.abort == kAbortCapacity ? kBlockFallback : LSG_OK;
 }
 
 int run_block_single(int algo, Workspace *w, const CsrView &g, uint32_t seed, const lsg_opts &o, uint32_t *dist,
-                     lsg_stats *st, int64_t *resume_cd) {
-  return algo == (int)Algo::SSSP ? run_block<Algo::SSSP>(w, g, &seed, 1, o, dist, st, resume_cd)
-                                 : run_block<Algo::BFS>(w, g, &seed, 1, o, dist, st, resume_cd);
+                     lsg_stats *st, Resume *resume) {
+  return algo == (int)Algo::SSSP ? run_block<Algo::SSSP>(w, g, &seed, 1, o, dist, st, resume)
+                                 : run_block<Algo::BFS>(w, g, &seed, 1, o, dist, st, resume);
 }
 
 int run_graphs_block(Workspace *w, const lsg_graph *graphs, uint32_t ngraphs, const uint32_t *srcs,

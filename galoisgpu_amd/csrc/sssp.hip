@@ -47,12 +47,12 @@ static KernelFn pick_kernel(int bs) {
 
 // Runs one traversal from `nseeds` seeds (host array) into dist (device,
 // g.nnodes entries).  Synchronous.  With graph_launch the init, seed and
-// persistent kernels are replayed as one captured hipGraph.  resume_cd >= 0:
-// continue from the distances already in dist instead (every reached vertex
-// queued, curdelta resume_cd); the seeds are then not used.
+// persistent kernels are replayed as one captured hipGraph.  With `resume`:
+// continue from the distances already in dist instead (resume_seed_kernel's
+// queue, curdelta resume->curdelta), not captured; the seeds are then not used.
 template <Algo A>
 static int run_core(Workspace *w, const CsrView &g, const uint32_t *seeds, uint32_t nseeds, const lsg_opts &o,
-                    uint32_t *dist, bool graph_launch, lsg_stats *st, int64_t resume_cd = -1) {
+                    uint32_t *dist, bool graph_launch, lsg_stats *st, const Resume *resume = nullptr) {
   auto t0 = std::chrono::steady_clock::now();
   int bs, rc;
   KArgs a;
@@ -77,9 +77,11 @@ static int run_core(Workspace *w, const CsrView &g, const uint32_t *seeds, uint3
     const uint32_t init_blocks = std::min<uint32_t>((g.nnodes + 255) / 256, 8u * w->cus);
     const uint32_t *sd = (const uint32_t *)seedbuf;
     auto enqueue = [&]() -> int {
-      if (resume_cd >= 0) {
-        hipLaunchKernelGGL(resume_init_kernel, dim3(init_blocks), dim3(256), 0, w->stream, a, a.delta, resume_cd);
-        hipLaunchKernelGGL(resume_seed_kernel, dim3(init_blocks), dim3(256), 0, w->stream, a);
+      if (resume) {
+        hipLaunchKernelGGL(resume_init_kernel, dim3(init_blocks), dim3(256), 0, w->stream, a, a.delta,
+                           resume->curdelta);
+        hipLaunchKernelGGL(resume_seed_kernel, dim3(init_blocks), dim3(256), 0, w->stream, a, resume->lo,
+                           resume->hi);
       } else {
         hipLaunchKernelGGL(init_kernel, dim3(init_blocks), dim3(256), 0, w->stream, a, a.delta);
         hipLaunchKernelGGL(seed_kernel, dim3(1), dim3(256), 0, w->stream, a, sd, nseeds);
@@ -89,7 +91,7 @@ static int run_core(Workspace *w, const CsrView &g, const uint32_t *seeds, uint3
       return LSG_OK;
     };
     HIPCHK(hipEventRecord(w->ev0, w->stream));
-    if (graph_launch && resume_cd < 0) {
+    if (graph_launch && !resume) {
       std::vector<uint8_t> key;
       key_append(key, (int)A);
       key_append(key, a);
@@ -165,12 +167,12 @@ static int run_single(const lsg_graph *g, const lsg_opts *o_in, uint32_t *dist, 
   if (o.single_block && o.enable_lb) return fail(LSG_ERR_ARG, "single_block and enable_lb (-l) exclude each other");
   if (o.single_block) {
     lsg_stats bst;
-    int64_t cd = 0;
-    rc = run_block_single((int)A, w, v, seed, o, dist, &bst, &cd);
+    Resume res;
+    rc = run_block_single((int)A, w, v, seed, o, dist, &bst, &res);
     if (rc == kBlockFallback) {
       // The piles overflowed: the persistent kernel continues from the
       // distances the workgroup reached (not captured: a one-off launch).
-      rc = run_core<A>(w, v, &seed, 1, o, dist, false, st, cd);
+      rc = run_core<A>(w, v, &seed, 1, o, dist, false, st, &res);
       if (rc == LSG_OK && st) {
         st->ms_kernel += bst.ms_kernel;
         st->items += bst.items;

@@ -1056,7 +1056,7 @@ static __global__ void init_kernel(KArgs a, uint32_t delta) {
 // Continuing a traversal that another kernel left part done (the one-workgroup
 // kernel whose piles overflowed): distances kept, stamps and control block
 // reset, curdelta at the threshold the traversal had reached.
-// resume_seed_kernel then queues the reached vertices.
+// resume_seed_kernel then queues the vertices to expand.
 static __global__ void resume_init_kernel(KArgs a, uint32_t delta, int64_t cd0) {
   uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
   for (uint32_t v = tid; v < a.nnodes; v += nt) {
@@ -1066,29 +1066,44 @@ static __global__ void resume_init_kernel(KArgs a, uint32_t delta, int64_t cd0) 
   ctrl_init(a, delta, cd0, tid, nt);
 }
 
-// Every reached vertex (d < INF) forms the first in-queue, so the traversal
-// continues exactly: every distance is the length of a path, and every vertex
-// whose distance may not yet have been passed on to its neighbours (a pending
-// pile entry, or an improvement whose push found the pile full) is expanded
-// again.  Settled vertices are expanded once more, for nothing.  The queue
-// order is unspecified; the distances do not depend on it.  256-thread
-// blocks, grid-strided in whole blocks; one queue reservation per block step.
-static __global__ __launch_bounds__(256) void resume_seed_kernel(KArgs a) {
+// Where a traversal continues after another kernel left it part done: the
+// vertices with lo <= d <= hi form the first in-queue, and distances above hi
+// are set back to INF.  SSSP (lo 0, hi INF - 1): every reached vertex is
+// queued, so the traversal continues exactly: every distance is the length of
+// a path, and every vertex whose distance may not yet have been passed on to
+// its neighbours (a pending pile entry, or an improvement whose push found
+// the pile full) is expanded again; settled vertices are expanded once more,
+// for nothing.  BFS, whose rounds are levels: lo = hi = the level that was
+// being expanded, which was complete and exact when its round began; the
+// next level, part found, is found again from it (queueing it as well would
+// mix two levels in one round, and every vertex first reached at too deep a
+// level is expanded twice).
+struct Resume {
+  int64_t curdelta;
+  uint32_t lo, hi;
+};
+
+// The queue order is unspecified; the distances do not depend on it.
+// 256-thread blocks, grid-strided in whole blocks; one queue reservation per
+// block step.
+static __global__ __launch_bounds__(256) void resume_seed_kernel(KArgs a, uint32_t lo, uint32_t hi) {
   __shared__ uint32_t cnt, base;
   Ctrl *c = a.ctrl;
   for (uint32_t v0 = blockIdx.x * 256u; v0 < a.nnodes; v0 += gridDim.x * 256u) {
     const uint32_t v = v0 + threadIdx.x;
-    const bool reached = v < a.nnodes && glb(a.dist)[v] != kInf;
+    const uint32_t d = v < a.nnodes ? glb(a.dist)[v] : kInf;
+    const bool queued = d >= lo && d <= hi;
+    if (d > hi && d != kInf) glb(a.dist)[v] = kInf;
     if (threadIdx.x == 0) cnt = 0;
     __syncthreads();
-    const uint32_t slot = reached ? atomicAdd(&cnt, 1u) : 0u;
+    const uint32_t slot = queued ? atomicAdd(&cnt, 1u) : 0u;
     __syncthreads();
     if (threadIdx.x == 0 && cnt) {
       base = atomicAdd(&c->qcount[0], cnt);
       atomicAdd(&c->s.in_count, cnt);
     }
     __syncthreads();
-    if (reached) glb(a.q[0])[base + slot] = v;
+    if (queued) glb(a.q[0])[base + slot] = v;
     __syncthreads();  // base and cnt are read before the next step resets them
   }
 }
@@ -1204,12 +1219,12 @@ int run_lb_single(int algo, Workspace *w, const CsrView &g, uint32_t seed, const
 
 // The one-workgroup near/far traversals (block.hip).  kBlockFallback: a pile
 // overflowed.  A single source then continues in the persistent kernel from
-// the distances left in dist, with curdelta *resume_cd (run_core's
-// resume_cd); a batch of graphs runs the union launch instead.  *st holds the
-// one-workgroup launch's figures either way.
+// the distances left in dist, as *resume says (run_core's resume); a batch of
+// graphs runs the union launch instead.  *st holds the one-workgroup launch's
+// figures either way.
 constexpr int kBlockFallback = 1;
 int run_block_single(int algo, Workspace *w, const CsrView &g, uint32_t seed, const lsg_opts &o, uint32_t *dist,
-                     lsg_stats *st, int64_t *resume_cd);
+                     lsg_stats *st, Resume *resume);
 int run_graphs_block(Workspace *w, const lsg_graph *graphs, uint32_t ngraphs, const uint32_t *srcs,
                      int32_t *const *dist_out, const lsg_opts &o, lsg_batch_stats *stats);
 
