@@ -11,7 +11,8 @@ oracle's serial near/far restatement timed beside it on the host.
       mode (single_block) at delta 32 and at the autotuner's delta, and BFS
   C2  RMAT-20 ef16, w in [1,255], max-degree source, delta 32 (SSSP and BFS)
   C3  2048x2048 road-like grid (10% edges removed, 0.1% shortcuts), w in
-      [1,10000], source 0, delta 10000: plain fixed delta vs learned schedule
+      [1,10000], source 0, delta 10000: plain fixed delta vs learned schedule,
+      and the first rounds in one workgroup (single_block)
   C4B 256 RMAT graphs, scales uniform in [10,16], ef 8, max-degree source each:
       one union launch vs the plain kernel graph by graph
   C5  RMAT-24 ef16, 32 sources: batched (+ learned) vs plain per source
@@ -175,6 +176,8 @@ def c3(repeats: int) -> dict:
     variants = [("plain", {"expand_mode": 1}), ("plain_twb", {"expand_mode": 0})]
     if have_model("plain"):
         variants += [("learned", {"sched": 1, "expand_mode": 1}), ("learned_twb", {"sched": 1, "expand_mode": 0})]
+    # the first rounds in one workgroup, the rest in the persistent kernel once the piles overflow
+    variants += [("single_block", {"single_block": True})]
     for name, kw in variants:
         o = Options(delta=10000, start_node=0, **kw)
         st = sssp_into(dev, buf.ptr, o)
