@@ -48,6 +48,15 @@ constexpr uint32_t kBlkThreads = 1024;
 constexpr uint32_t kPileCap = 2048;  // LDS: entries per round pile (two of them: 64 KB)
 constexpr uint32_t kFarCap = 4096;   // LDS: entries in the far pile (64 KB)
 constexpr uint32_t kFarPerThread = kFarCap / kBlkThreads;
+// A BFS level wider than this continues in the persistent kernel: such a
+// level is faster over the whole GPU than in one CU, and levels only widen
+// on the graphs that reach it (BFS from source 0, road-like graphs of 256^2
+// to 2048^2, switch at 2048 / 1024 / 512 entries: 256^2 0.98 / 0.85 / 0.82
+// ms, 1024^2 1.39 / 1.31 / 1.29, persistent kernel alone 0.92 and 1.45; 1024
+// keeps the levels of grids up to 1024^2 in the workgroup).  SSSP rounds
+// switch only when the pile is full: at 512 entries the 256^2 grid at delta
+// 256 took 3.9 instead of 2.6 ms, at 1024 no faster than at 2048.
+constexpr uint32_t kBfsLevelCap = 1024;
 
 // Relaxes kInFlight edges per lane: the atomicMins and the neighbours' row
 // bounds are issued together, then every improved neighbour is pushed near or
@@ -179,8 +188,8 @@ __global__ __launch_bounds__(kBlkThreads) void block_kernel(KArgs a) {
   for (;;) {
     if (rs.state == kDone) break;
     const uint32_t cur = rs.cur, nxt = cur ^ 1u;
-    const Sink<kLdsAS> s{(EntryPtr<kLdsAS>)pile[nxt], &rs.count[nxt], kPileCap, (EntryPtr<kLdsAS>)far,
-                         &rs.far_count, kFarCap, &rs.abort};
+    const Sink<kLdsAS> s{(EntryPtr<kLdsAS>)pile[nxt], &rs.count[nxt], A == Algo::BFS ? kBfsLevelCap : kPileCap,
+                         (EntryPtr<kLdsAS>)far, &rs.far_count, kFarCap, &rs.abort};
     expand_round<A, kLdsAS>(a, (EntryPtr<kLdsAS>)pile[cur], rs.count[cur], rs.thr, s, &rs.items);
     __syncthreads();  // every push of the round is in
     end_round(rs);

@@ -105,7 +105,8 @@ typedef struct lsg_opts {
                            high-diameter graphs, where a round of the persistent kernel is a chain
                            of global round trips: SSSP on the 256x256 grid 8.0 -> 4.2 ms at delta
                            32, 2.5 ms at delta 256; BFS 3.6 -> 1.5 ms.  When the piles outgrow
-                           LDS (2048 near, 4096 far entries) the persistent kernel continues from
+                           LDS (SSSP: 2048 entries per round, 4096 far; BFS levels wider than
+                           1024 vertices) the persistent kernel continues from
                            the distances reached (road-like 1024x1024 at delta 10000: 3.4 -> 3.0
                            ms); a frontier that explodes at once (RMAT BFS from a hub) costs
                            that launch extra.  stats then add both launches' rounds, items and

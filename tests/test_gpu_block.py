@@ -92,6 +92,9 @@ def test_single_block_roadlike(rows, seed):
     say("single block road", rows)
     d, _ = run(g, 0, delta=10000)
     assert np.array_equal(d, ref)
+    lref, _ = oracle.bfs(g.row_start, g.edge_dst, 0)
+    b, _ = run(g, 0, bfs=True)
+    assert np.array_equal(b, lref)
 
 
 def test_single_block_rmat_and_zero_weights():
