@@ -390,6 +390,12 @@ static int run_block(Workspace *w, const CsrView &g, const uint32_t *seeds, uint
   HIPCHK(hipStreamSynchronize(w->stream));
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, w->ev0, w->ev1));
+  // The workgroup records no round trace: a traced call's trace holds the
+  // persistent kernel's rounds after an overflow, and is empty otherwise.
+  if (o.trace) {
+    w->last_trace.clear();
+    w->last_feat.clear();
+  }
   if (st) {
     memset(st, 0, sizeof *st);
     st->ms_kernel = ms;

@@ -106,11 +106,12 @@ typedef struct lsg_opts {
                            of global round trips: SSSP on the 256x256 grid 8.0 -> 4.2 ms at delta
                            32, 2.5 ms at delta 256; BFS 3.6 -> 1.5 ms.  When the piles outgrow
                            LDS (SSSP: 2048 entries per round, 4096 far; BFS levels wider than
-                           1024 vertices) the persistent kernel continues from
-                           the distances reached (road-like 1024x1024 at delta 10000: 3.4 -> 3.0
-                           ms); a frontier that explodes at once (RMAT BFS from a hub) costs
-                           that launch extra.  stats then add both launches' rounds, items and
-                           kernel time.  Single-source lsg_sssp_dist / lsg_bfs_dist / gg_main only */
+                           1024 vertices) the persistent kernel continues from the distances
+                           reached (road-like 1024x1024 at delta 10000: 3.4 -> 3.0 ms); a
+                           frontier that explodes at once (RMAT BFS from a hub) costs that launch
+                           extra.  stats then add both launches' rounds, items and kernel time;
+                           a trace holds the persistent kernel's rounds only.  Single-source
+                           lsg_sssp_dist / lsg_bfs_dist / gg_main only */
   int32_t reserved[2];
 } lsg_opts;
 

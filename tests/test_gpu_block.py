@@ -31,7 +31,7 @@ def run(g, src, bfs=False, **kw):
     dev = g.copy_to_gpu()
     buf = DeviceArray(g.nnodes * 4)
     try:
-        o = Options(start_node=src, single_block=True, **kw)
+        o = Options(start_node=src, **{"single_block": True, **kw})
         st = (bfs_into if bfs else sssp_into)(dev, buf.ptr, o)
         return buf.to_numpy(np.int32, g.nnodes), st
     finally:
@@ -263,3 +263,21 @@ def test_single_block_overflow_continues_in_the_persistent_kernel():
         b, _ = run(g, src, bfs=True)
         assert np.array_equal(b, lref), (k, "bfs")
     assert fell_back >= len(cases) // 2, fell_back
+
+
+def test_single_block_trace_holds_the_persistent_rounds_only():
+    """A traced one-workgroup call leaves no round trace when it completes in the workgroup (not
+    the previous call's), and the persistent kernel's rounds when it continued there."""
+    from galoisgpu_amd.apps import last_trace
+    g = CSRGraph.grid(64, 64, 1, 100, seed=5)
+    say("single block trace")
+    run(g, 0, delta=32, single_block=False, trace=True)  # a persistent-kernel trace first
+    assert len(last_trace()[0]) > 0
+    d, st = run(g, 0, delta=32, trace=True)
+    assert st["grid_blocks"] == 1 and len(last_trace()[0]) == 0
+    assert np.array_equal(d, oracle.dijkstra(g.row_start, g.edge_dst, g.edge_data, 0))
+    r = CSRGraph.rmat(14, 8, seed=9)
+    src = r.max_degree_vertex()
+    d, st = run(r, src, delta=32, trace=True)
+    assert st["grid_blocks"] > 1 and len(last_trace()[0]) > 0
+    assert np.array_equal(d, oracle.dijkstra(r.row_start, r.edge_dst, r.edge_data, src))
