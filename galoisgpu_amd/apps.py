@@ -54,6 +54,8 @@ def process_prog_opt(opts: Options, c: str, optarg: Optional[str] = None, app: s
         opts.delta = v
     elif c == "l":
         opts.enable_lb = True
+    elif c == "w":  # no reference counterpart: the one-workgroup first rounds (single_block)
+        opts.single_block = True
     elif c == "s":
         v = int(optarg)
         if v < 0:

@@ -27,12 +27,12 @@
 #include "lsg.h"
 
 #if defined(LSG_APP_SSSP)
-static const char *prog_opts = "ls:d:";
-static const char *prog_usage = "[-l] [-d delta] [-s startNode]";
+static const char *prog_opts = "lws:d:";
+static const char *prog_usage = "[-l] [-w] [-d delta] [-s startNode]";
 static const char *GGC_OPTIONS = "lsg-mi355x sssp: persistent near/far, wave64, gfx950";
 #elif defined(LSG_APP_BFS)
-static const char *prog_opts = "lDs:";
-static const char *prog_usage = "[-l] [-D] [-s startNode]";
+static const char *prog_opts = "lwDs:";
+static const char *prog_usage = "[-l] [-w] [-D] [-s startNode]";
 static const char *GGC_OPTIONS = "lsg-mi355x bfs: persistent level-synchronous, wave64, gfx950";
 #elif defined(LSG_APP_CC)
 static const char *prog_opts = "";
@@ -50,9 +50,12 @@ static const char *prog_args_usage = "";
 #elif defined(LSG_APP_BFS)
 static const char *prog_args_usage =
     "-l: enable thread block load balancer (by default false)\n"
+    " -w: the first levels in one workgroup, until one is wider than it takes (small high-diameter graphs)\n"
     " -D: direction-optimising BFS over a reverse CSR built on the GPU before the timed region";
 #else
-static const char *prog_args_usage = "-l: enable thread block load balancer (by default false)";
+static const char *prog_args_usage =
+    "-l: enable thread block load balancer (by default false)\n"
+    " -w: the first rounds in one workgroup, until its piles overflow (small high-diameter graphs)";
 #endif
 static int DIRECTION_OPT = 0;  // bfs -D
 #if defined(LSG_APP_PR)
@@ -84,6 +87,7 @@ static void process_prog_opt(char c, char *optarg) {
     }
   }
   if (c == 'l') OPTS.enable_lb = 1;
+  if (c == 'w') OPTS.single_block = 1;  // no reference counterpart: lsg_opts.single_block
   if (c == 'D') DIRECTION_OPT = 1;
 #if defined(LSG_APP_PR)
   if (c == 'n') NO_PRINT_PAGERANK = 1;
@@ -124,6 +128,10 @@ static void parse_args(int argc, char *argv[]) {
   if (optind < argc) INPUT = argv[optind];
   else {
     usage(argc, argv);
+    exit(EXIT_FAILURE);
+  }
+  if (OPTS.single_block && (OPTS.enable_lb || DIRECTION_OPT)) {
+    fprintf(stderr, "-w excludes -l and -D\n");
     exit(EXIT_FAILURE);
   }
 }

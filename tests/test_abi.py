@@ -55,6 +55,8 @@ def test_prog_opts():
     process_prog_opt(o, "s", "5")
     process_prog_opt(o, "l")
     assert (o.delta, o.start_node, o.enable_lb) == (32, 5, True)
+    process_prog_opt(o, "w")
+    assert o.single_block
     with pytest.raises(ValueError):
         process_prog_opt(o, "d", "0")
     with pytest.raises(ValueError):
@@ -78,6 +80,8 @@ def test_cli_usage_errors():
     assert r.returncode == 1 and "usage:" in r.stderr
     r = subprocess.run([exe, "-d", "0", "g.gr"], capture_output=True, text=True)
     assert r.returncode == 1 and "delta" in r.stderr
+    r = subprocess.run([exe, "-l", "-w", "g.gr"], capture_output=True, text=True)
+    assert r.returncode == 1 and "-w excludes" in r.stderr
     exe_bfs = os.path.join(ROOT, "galoisgpu_amd", "bin", "bfs")
     r = subprocess.run([exe_bfs, "-d", "5", "g.gr"], capture_output=True, text=True)
     assert r.returncode == 1 and "usage:" in r.stderr  # bfs has no -d (apps/bfs/support.cu:5)

@@ -250,6 +250,17 @@ def test_cli_output_matches_oracle_text(tmp_path):
     lv, _ = oracle.bfs(g.row_start, g.edge_dst, 5)
     want = "".join(("%d INF\n" % i) if x == INF else ("%d %d\n" % (i, x)) for i, x in enumerate(lv))
     assert open(out2).read() == want
+    say("cli -w")
+    out4 = str(tmp_path / "bfs_w.txt")
+    r = subprocess.run([exe, "-w", "-s", "5", "-o", out4, gr], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert open(out4).read() == want
+    out5 = str(tmp_path / "sssp_w.txt")
+    r = subprocess.run([os.path.join(ROOT, "galoisgpu_amd", "bin", "sssp"), "-w", "-d", "32", "-o", out5, gr],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert open(out5).read() == "".join(("%d INF\n" % i) if x == INF else ("%d %d\n" % (i, x))
+                                        for i, x in enumerate(ref))
     say("cli bfs -D")
     out3 = str(tmp_path / "bfs_do.txt")
     r = subprocess.run([exe, "-D", "-s", "5", "-o", out3, gr], capture_output=True, text=True, timeout=120)
