@@ -281,3 +281,30 @@ def test_single_block_trace_holds_the_persistent_rounds_only():
     d, st = run(r, src, delta=32, trace=True)
     assert st["grid_blocks"] > 1 and len(last_trace()[0]) > 0
     assert np.array_equal(d, oracle.dijkstra(r.row_start, r.edge_dst, r.edge_data, src))
+
+
+def test_single_block_overflow_continues_under_learned_schedules():
+    """After an overflow the persistent kernel may run the learned scheduler (single_block with
+    sched): under every forced per-phase action, random schedules and an untrained network, the
+    distances equal the oracle's."""
+    from galoisgpu_amd import sched as lsched
+    base = lsched.new_model(0, kind="plain")
+    rng = np.random.default_rng(5)
+    schedules = [[a] for a in range(lsched.N_ACT)] + [list(rng.integers(0, lsched.N_ACT, 300)) for _ in range(2)]
+    cases = [(CSRGraph.roadlike(256, 256, 0.10, 0.001, 1, 10000, seed=3), 0, 10000),
+             (CSRGraph.rmat(14, 8, seed=9), None, 32)]
+    try:
+        for g, src, delta in cases:
+            src = g.max_degree_vertex() if src is None else src
+            ref = oracle.dijkstra(g.row_start, g.edge_dst, g.edge_data, src)
+            for sch in schedules:
+                lsched.set_model(lsched.forced(base, sch), kind="plain")
+                say("single block then learned", g.nnodes, sch[:6])
+                d, st = run(g, src, delta=delta, sched=1)
+                assert np.array_equal(d, ref), sch
+                assert st["grid_blocks"] > 1  # the workgroup overflowed and the persistent kernel went on
+            lsched.set_model(base, kind="plain")
+            d, _ = run(g, src, delta=delta, sched=1)
+            assert np.array_equal(d, ref)
+    finally:
+        lsched.set_model(base, kind="plain")
