@@ -859,8 +859,10 @@ __device__ __forceinline__ void expand_entries(const KArgs &a, EntryPtr<AS> in, 
     Entry e{0u, kInf, 0u, 0u};
     if (valid) e = ld_entry<AS>(in + i);
     // issued by every lane (a load under a per-lane condition makes the
-    // compiler wait for it before the edge loads); d[0] for idle lanes
-    const uint32_t du = ld_coh(a.dist + e.v);
+    // compiler wait for it before the edge loads); d[0] for idle lanes.  A
+    // BFS entry is never stale (its level is exact when it is pushed, and a
+    // vertex is pushed once), so BFS takes the entry's level without a load.
+    const uint32_t du = A == Algo::BFS ? e.d : ld_coh(a.dist + e.v);
     const uint32_t deg = valid ? e.re - e.rs : 0u;
     const bool heavy = deg > kEntryHeavyDeg;
     const uint32_t ldeg = heavy ? 0u : deg;
